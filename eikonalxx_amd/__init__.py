@@ -1,0 +1,549 @@
+"""eikonalxx_amd -- MI355X (gfx950) fast-sweeping eikonal solver.
+
+Python mirror of the reference's solver interface (uofuseismo/eikonalxx:
+include/eikonalxx/solver3d.hpp, solver2d.hpp, solverOptions.hpp,
+geometry3d.hpp; the reference's own Python bindings are python/pySolver2d.cpp)
+over the C ABI of libeikonalxx_amd.so (include/eikonalxx_amd.h).  Errors map
+as in the reference: invalid arguments raise ValueError
+(std::invalid_argument), call-order violations raise RuntimeError
+(std::runtime_error).
+
+There is no CPU fallback: every solve runs the HIP kernels, and loading the
+package fails loudly when the shared library has not been built.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libeikonalxx_amd.so")
+
+__all__ = ["SolverAlgorithm", "Precision", "Schedule", "Kernel", "Ordering",
+           "SolverOptions", "Geometry3D", "Geometry2D", "Solver3D",
+           "Batch3D", "Solver2D", "SolveStats", "library", "device_count",
+           "synchronize", "DeviceError"]
+
+
+class SolverAlgorithm(enum.IntEnum):
+    """include/eikonalxx/enums.hpp:44-52"""
+    LevelSetMethod = 0
+    FastSweepingMethod = 1
+
+
+class Precision(enum.IntEnum):
+    FP64 = 0
+    FP32_POLISH = 1
+    FP32 = 2
+
+
+class Schedule(enum.IntEnum):
+    COMPLETED = 0
+    SHIPPED = 1
+
+
+class Kernel(enum.IntEnum):
+    AUTO = 0
+    HYPERPLANE = 1
+    COLUMN = 2
+
+
+class Ordering(enum.IntEnum):
+    """include/eikonalxx/enums.hpp:7-40"""
+    Natural = 0
+    XYZ = 1
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure (EKX_EHIP / EKX_ENOMEM)."""
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("number_of_sweeps", ctypes.c_int),
+                ("factored_radius", ctypes.c_int),
+                ("algorithm", ctypes.c_int),
+                ("verbosity", ctypes.c_int),
+                ("convergence_tolerance", ctypes.c_double),
+                ("precision", ctypes.c_int),
+                ("schedule", ctypes.c_int),
+                ("kernel", ctypes.c_int),
+                ("early_exit", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+class _Geometry3D(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double),
+                ("z0", ctypes.c_double)]
+
+
+class _Geometry2D(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x0", ctypes.c_double), ("z0", ctypes.c_double)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("updates", ctypes.c_int64),
+                ("updates_fp64", ctypes.c_int64),
+                ("kernel_launches", ctypes.c_int64),
+                ("iterations_run", ctypes.c_int32),
+                ("reserved", ctypes.c_int32),
+                ("last_solve_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_
+                if name != "reserved"}
+
+
+class _Problem3D(ctypes.Structure):
+    _fields_ = [("geometry", _Geometry3D),
+                ("slowness", ctypes.POINTER(ctypes.c_double)),
+                ("source", ctypes.c_double * 3)]
+
+
+# Every symbol declared in include/eikonalxx_amd.h.
+EXPORTED_SYMBOLS = [
+    "ekx_last_error", "ekx_version", "ekx_solver_options_default",
+    "ekx_device_count", "ekx_device_synchronize",
+    "ekx_solver3d_create", "ekx_solver3d_destroy", "ekx_solver3d_initialize",
+    "ekx_solver3d_is_initialized", "ekx_solver3d_set_slowness",
+    "ekx_solver3d_set_nodal_velocities", "ekx_solver3d_set_cell_velocities",
+    "ekx_solver3d_have_velocity_model", "ekx_solver3d_set_source",
+    "ekx_solver3d_have_source", "ekx_solver3d_get_source_cell",
+    "ekx_solver3d_solve", "ekx_solver3d_have_travel_time_field",
+    "ekx_solver3d_get_travel_times", "ekx_solver3d_get_travel_times_f32",
+    "ekx_solver3d_get_travel_times_device", "ekx_solver3d_get_stats",
+    "ekx_solver3d_get_stream",
+    "ekx_batch3d_create", "ekx_batch3d_destroy", "ekx_batch3d_initialize",
+    "ekx_batch3d_set_source", "ekx_batch3d_solve",
+    "ekx_batch3d_get_travel_times", "ekx_batch3d_get_stats",
+    "ekx_batch3d_get_stream",
+    "ekx_solver2d_create", "ekx_solver2d_destroy", "ekx_solver2d_initialize",
+    "ekx_solver2d_set_slowness", "ekx_solver2d_set_nodal_velocities",
+    "ekx_solver2d_set_source", "ekx_solver2d_set_stations",
+    "ekx_solver2d_solve", "ekx_solver2d_get_travel_times",
+    "ekx_solver2d_get_travel_times_at_stations", "ekx_solver2d_get_stats",
+]
+
+_lib = None
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_PD = ctypes.POINTER(ctypes.c_double)
+
+
+def library():
+    """Loads libeikonalxx_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(
+                "libeikonalxx_amd.so is not built; run "
+                "'python -m eikonalxx_amd.build' (the solver has no CPU "
+                "fallback)")
+        lib = ctypes.CDLL(LIBRARY_PATH)
+        _declare(lib)
+        _lib = lib
+    return _lib
+
+
+def _declare(L):
+    L.ekx_last_error.restype = ctypes.c_char_p
+    L.ekx_version.restype = ctypes.c_char_p
+    L.ekx_solver_options_default.restype = None
+    L.ekx_solver_options_default.argtypes = [ctypes.POINTER(_Options)]
+    L.ekx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    for name in ("ekx_solver3d_destroy", "ekx_batch3d_destroy",
+                 "ekx_solver2d_destroy"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [_P]
+    for name in ("ekx_solver3d_create", "ekx_batch3d_create",
+                 "ekx_solver2d_create"):
+        getattr(L, name).argtypes = [ctypes.POINTER(_P)]
+    L.ekx_solver3d_initialize.argtypes = [_P, ctypes.POINTER(_Options),
+                                          ctypes.POINTER(_Geometry3D)]
+    L.ekx_solver3d_set_slowness.argtypes = [_P, _I64, _PD]
+    L.ekx_solver3d_set_nodal_velocities.argtypes = [_P, _I64, _PD, _I]
+    L.ekx_solver3d_set_cell_velocities.argtypes = [_P, _I64, _PD, _I]
+    L.ekx_solver3d_set_source.argtypes = [_P, _D, _D, _D]
+    L.ekx_solver3d_get_source_cell.argtypes = [
+        _P, _PD, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.ekx_solver3d_solve.argtypes = [_P]
+    L.ekx_solver3d_get_travel_times.argtypes = [_P, _I64, _PD]
+    L.ekx_solver3d_get_travel_times_f32.argtypes = [
+        _P, _I64, ctypes.POINTER(ctypes.c_float)]
+    L.ekx_solver3d_get_travel_times_device.argtypes = [
+        _P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int)]
+    L.ekx_solver3d_get_stats.argtypes = [_P, ctypes.POINTER(SolveStats)]
+    L.ekx_solver3d_get_stream.argtypes = [_P, ctypes.POINTER(_P)]
+    for name in ("ekx_solver3d_is_initialized",
+                 "ekx_solver3d_have_velocity_model",
+                 "ekx_solver3d_have_source",
+                 "ekx_solver3d_have_travel_time_field"):
+        getattr(L, name).argtypes = [_P]
+    L.ekx_batch3d_initialize.argtypes = [_P, ctypes.POINTER(_Options), _I,
+                                         ctypes.POINTER(_Problem3D)]
+    L.ekx_batch3d_set_source.argtypes = [_P, _I, _D, _D, _D]
+    L.ekx_batch3d_solve.argtypes = [_P]
+    L.ekx_batch3d_get_travel_times.argtypes = [_P, _I, _I64, _PD]
+    L.ekx_batch3d_get_stats.argtypes = [_P, ctypes.POINTER(SolveStats)]
+    L.ekx_batch3d_get_stream.argtypes = [_P, ctypes.POINTER(_P)]
+    L.ekx_solver2d_initialize.argtypes = [_P, ctypes.POINTER(_Options),
+                                          ctypes.POINTER(_Geometry2D)]
+    L.ekx_solver2d_set_slowness.argtypes = [_P, _I64, _PD]
+    L.ekx_solver2d_set_nodal_velocities.argtypes = [_P, _I64, _PD, _I]
+    L.ekx_solver2d_set_source.argtypes = [_P, _D, _D]
+    L.ekx_solver2d_set_stations.argtypes = [_P, _I, _PD, _PD]
+    L.ekx_solver2d_solve.argtypes = [_P]
+    L.ekx_solver2d_get_travel_times.argtypes = [_P, _I64, _PD]
+    L.ekx_solver2d_get_travel_times_at_stations.argtypes = [_P, _I, _PD]
+    L.ekx_solver2d_get_stats.argtypes = [_P, ctypes.POINTER(SolveStats)]
+
+
+def _check(status):
+    if status == 0:
+        return
+    msg = library().ekx_last_error().decode()
+    if status == 1:
+        raise ValueError(msg)
+    if status == 2:
+        raise RuntimeError(msg)
+    raise DeviceError(msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(library().ekx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def synchronize():
+    _check(library().ekx_device_synchronize())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_PD)
+
+
+class SolverOptions:
+    """SolverOptions (include/eikonalxx/solverOptions.hpp:54-97) plus the
+    device choices.  Defaults follow src/solverOptions.cpp:5-21."""
+
+    def __init__(self, number_of_sweeps=None, factored_radius=None,
+                 algorithm=None, precision=Precision.FP64,
+                 schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
+                 convergence_tolerance=None, early_exit=False,
+                 use_graph=True, verbosity=0):
+        o = _Options()
+        library().ekx_solver_options_default(ctypes.byref(o))
+        if number_of_sweeps is not None:
+            o.number_of_sweeps = int(number_of_sweeps)
+        if factored_radius is not None:
+            o.factored_radius = int(factored_radius)
+        if algorithm is not None:
+            o.algorithm = int(algorithm)
+        if convergence_tolerance is not None:
+            o.convergence_tolerance = float(convergence_tolerance)
+        o.precision = int(precision)
+        o.schedule = int(schedule)
+        o.kernel = int(kernel)
+        o.early_exit = 1 if early_exit else 0
+        o.use_graph = 1 if use_graph else 0
+        o.verbosity = int(verbosity)
+        self._o = o
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        return getattr(self._o, name)
+
+    # reference-style accessors
+    def set_number_of_sweeps(self, n):
+        self._o.number_of_sweeps = int(n)
+
+    def get_number_of_sweeps(self):
+        return self._o.number_of_sweeps
+
+    def set_factored_eikonal_equation_solver_radius(self, r):
+        self._o.factored_radius = int(r)
+
+    def get_factored_eikonal_equation_solver_radius(self):
+        return self._o.factored_radius
+
+    def set_algorithm(self, a):
+        self._o.algorithm = int(a)
+
+    def get_algorithm(self):
+        return SolverAlgorithm(self._o.algorithm)
+
+
+class Geometry3D:
+    """Geometry3D (include/eikonalxx/geometry3d.hpp:50-160)."""
+
+    def __init__(self, nx, ny, nz, dx, dy=None, dz=None, x0=0.0, y0=0.0,
+                 z0=0.0):
+        dy = dx if dy is None else dy
+        dz = dx if dz is None else dz
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.x0, self.y0, self.z0 = float(x0), float(y0), float(z0)
+
+    @property
+    def n_grid(self):
+        return self.nx * self.ny * self.nz
+
+    @property
+    def n_cells(self):
+        return (self.nx - 1) * (self.ny - 1) * (self.nz - 1)
+
+    def _c(self):
+        return _Geometry3D(self.nx, self.ny, self.nz, self.dx, self.dy,
+                           self.dz, self.x0, self.y0, self.z0)
+
+
+class Geometry2D:
+    """Geometry2D (include/eikonalxx/geometry2d.hpp)."""
+
+    def __init__(self, nx, nz, dx, dz=None, x0=0.0, z0=0.0):
+        dz = dx if dz is None else dz
+        self.nx, self.nz = int(nx), int(nz)
+        self.dx, self.dz = float(dx), float(dz)
+        self.x0, self.z0 = float(x0), float(z0)
+
+    @property
+    def n_grid(self):
+        return self.nx * self.nz
+
+    @property
+    def n_cells(self):
+        return (self.nx - 1) * (self.nz - 1)
+
+    def _c(self):
+        return _Geometry2D(self.nx, self.nz, self.dx, self.dz, self.x0,
+                           self.z0)
+
+
+class Solver3D:
+    """EikonalXX::Solver3D<double> (include/eikonalxx/solver3d.hpp:19-177)
+    on the GPU."""
+
+    def __init__(self):
+        self._L = library()
+        self._h = _P()
+        _check(self._L.ekx_solver3d_create(ctypes.byref(self._h)))
+        self._geometry = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.ekx_solver3d_destroy(h)
+            self._h = None
+
+    def initialize(self, options, geometry):
+        g = geometry._c()
+        _check(self._L.ekx_solver3d_initialize(self._h,
+                                               ctypes.byref(options._o),
+                                               ctypes.byref(g)))
+        self._geometry = geometry
+
+    def is_initialized(self):
+        return bool(self._L.ekx_solver3d_is_initialized(self._h))
+
+    def set_slowness(self, slowness):
+        s = np.ascontiguousarray(slowness, dtype=np.float64).reshape(-1)
+        _check(self._L.ekx_solver3d_set_slowness(self._h, s.size, _dp(s)))
+
+    def set_nodal_velocities(self, velocity, ordering=Ordering.Natural):
+        v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
+        _check(self._L.ekx_solver3d_set_nodal_velocities(
+            self._h, v.size, _dp(v), int(ordering)))
+
+    def set_cell_velocities(self, velocity, ordering=Ordering.Natural):
+        v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
+        _check(self._L.ekx_solver3d_set_cell_velocities(
+            self._h, v.size, _dp(v), int(ordering)))
+
+    def have_velocity_model(self):
+        return bool(self._L.ekx_solver3d_have_velocity_model(self._h))
+
+    def set_source(self, location):
+        x, y, z = location
+        _check(self._L.ekx_solver3d_set_source(self._h, float(x), float(y),
+                                               float(z)))
+
+    def have_source(self):
+        return bool(self._L.ekx_solver3d_have_source(self._h))
+
+    def get_source_cell(self):
+        off = np.zeros(3)
+        cell = (ctypes.c_int * 3)()
+        idx = ctypes.c_int(0)
+        _check(self._L.ekx_solver3d_get_source_cell(self._h, _dp(off), cell,
+                                                    ctypes.byref(idx)))
+        return off, tuple(cell), idx.value
+
+    def solve(self):
+        _check(self._L.ekx_solver3d_solve(self._h))
+
+    def have_travel_time_field(self):
+        return bool(self._L.ekx_solver3d_have_travel_time_field(self._h))
+
+    def get_travel_time_field(self):
+        n = self._geometry.n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_solver3d_get_travel_times(self._h, n, _dp(out)))
+        return out
+
+    def get_travel_time_field_f32(self):
+        n = self._geometry.n_grid
+        out = np.empty(n, dtype=np.float32)
+        _check(self._L.ekx_solver3d_get_travel_times_f32(
+            self._h, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def get_stats(self):
+        st = SolveStats()
+        _check(self._L.ekx_solver3d_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def get_stream(self):
+        s = _P()
+        _check(self._L.ekx_solver3d_get_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+
+class Batch3D:
+    """Many independent 3D solves in one launch schedule.  ``problems`` is a
+    list of (Geometry3D, slowness array, (x, y, z)).  Problems passing the
+    same slowness array object share one device copy of the model."""
+
+    def __init__(self):
+        self._L = library()
+        self._h = _P()
+        _check(self._L.ekx_batch3d_create(ctypes.byref(self._h)))
+        self._keep = []
+        self._geoms = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.ekx_batch3d_destroy(h)
+            self._h = None
+
+    def initialize(self, options, problems):
+        arr = (_Problem3D * len(problems))()
+        keep, seen = [], {}
+        for i, (geom, slow, src) in enumerate(problems):
+            key = id(slow)
+            if key not in seen:
+                s = np.ascontiguousarray(slow, dtype=np.float64).reshape(-1)
+                if s.size != geom.n_cells:
+                    raise ValueError("problem %d: slowness has %d cells, "
+                                     "geometry needs %d"
+                                     % (i, s.size, geom.n_cells))
+                seen[key] = s
+                keep.append(s)
+            s = seen[key]
+            arr[i].geometry = geom._c()
+            arr[i].slowness = _dp(s)
+            arr[i].source[0], arr[i].source[1], arr[i].source[2] = src
+        _check(self._L.ekx_batch3d_initialize(self._h,
+                                              ctypes.byref(options._o),
+                                              len(problems), arr))
+        self._keep = keep
+        self._geoms = [p[0] for p in problems]
+
+    def set_source(self, problem, location):
+        x, y, z = location
+        _check(self._L.ekx_batch3d_set_source(self._h, int(problem),
+                                              float(x), float(y), float(z)))
+
+    def solve(self):
+        _check(self._L.ekx_batch3d_solve(self._h))
+
+    def get_travel_time_field(self, problem):
+        n = self._geoms[problem].n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_batch3d_get_travel_times(self._h, int(problem), n,
+                                                    _dp(out)))
+        return out
+
+    def get_stats(self):
+        st = SolveStats()
+        _check(self._L.ekx_batch3d_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def get_stream(self):
+        s = _P()
+        _check(self._L.ekx_batch3d_get_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+
+class Solver2D:
+    """EikonalXX::Solver2D<double> (include/eikonalxx/solver2d.hpp) with the
+    fast-sweeping schedule (src/solver2d.cpp:462-565) on the GPU."""
+
+    def __init__(self):
+        self._L = library()
+        self._h = _P()
+        _check(self._L.ekx_solver2d_create(ctypes.byref(self._h)))
+        self._geometry = None
+        self._n_stations = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.ekx_solver2d_destroy(h)
+            self._h = None
+
+    def initialize(self, options, geometry):
+        g = geometry._c()
+        _check(self._L.ekx_solver2d_initialize(self._h,
+                                               ctypes.byref(options._o),
+                                               ctypes.byref(g)))
+        self._geometry = geometry
+
+    def set_slowness(self, slowness):
+        s = np.ascontiguousarray(slowness, dtype=np.float64).reshape(-1)
+        _check(self._L.ekx_solver2d_set_slowness(self._h, s.size, _dp(s)))
+
+    def set_nodal_velocities(self, velocity, ordering=Ordering.Natural):
+        v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
+        _check(self._L.ekx_solver2d_set_nodal_velocities(
+            self._h, v.size, _dp(v), int(ordering)))
+
+    def set_source(self, location):
+        x, z = location
+        _check(self._L.ekx_solver2d_set_source(self._h, float(x), float(z)))
+
+    def set_stations(self, locations):
+        xs = np.ascontiguousarray([p[0] for p in locations], dtype=np.float64)
+        zs = np.ascontiguousarray([p[1] for p in locations], dtype=np.float64)
+        _check(self._L.ekx_solver2d_set_stations(self._h, len(locations),
+                                                 _dp(xs), _dp(zs)))
+        self._n_stations = len(locations)
+
+    def solve(self):
+        _check(self._L.ekx_solver2d_solve(self._h))
+
+    def get_travel_time_field(self):
+        n = self._geometry.n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_solver2d_get_travel_times(self._h, n, _dp(out)))
+        return out
+
+    def get_travel_times_at_stations(self):
+        out = np.empty(self._n_stations, dtype=np.float64)
+        _check(self._L.ekx_solver2d_get_travel_times_at_stations(
+            self._h, self._n_stations, _dp(out)))
+        return out
+
+    def get_stats(self):
+        st = SolveStats()
+        _check(self._L.ekx_solver2d_get_stats(self._h, ctypes.byref(st)))
+        return st

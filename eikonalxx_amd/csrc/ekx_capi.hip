@@ -1,0 +1,1141 @@
+// ekx_capi.hip -- host implementation of the C ABI declared in
+// include/eikonalxx_amd.h.  Owns device memory, builds the launch schedule
+// of a solve and replays it from a captured hipGraph.
+#include "eikonalxx_amd.h"
+#include "ekx_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace ekx;
+
+namespace
+{
+
+thread_local std::string g_error;
+
+ekx_status fail(ekx_status status, const std::string &message)
+{
+    g_error = message;
+    return status;
+}
+
+#define EKX_HIP(expr)                                                        \
+    do                                                                       \
+    {                                                                        \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+        {                                                                    \
+            return fail(EKX_EHIP, std::string(#expr) + ": " +                \
+                                      hipGetErrorString(e_));                \
+        }                                                                    \
+    } while (0)
+
+#define EKX_CHECK(expr)                                                      \
+    do                                                                       \
+    {                                                                        \
+        ekx_status s_ = (expr);                                              \
+        if (s_ != EKX_OK) { return s_; }                                     \
+    } while (0)
+
+template <typename T> struct DeviceBuffer
+{
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void release()
+    {
+        if (ptr) { (void) hipFree(ptr); }
+        ptr = nullptr;
+        count = 0;
+    }
+    ekx_status allocate(size_t n)
+    {
+        if (n == count && ptr) { return EKX_OK; }
+        release();
+        if (n == 0) { return EKX_OK; }
+        hipError_t e = hipMalloc((void **) &ptr, n*sizeof(T));
+        if (e != hipSuccess)
+        {
+            ptr = nullptr;
+            return fail(EKX_ENOMEM, "hipMalloc of " +
+                                        std::to_string(n*sizeof(T)) +
+                                        " bytes failed: " +
+                                        hipGetErrorString(e));
+        }
+        count = n;
+        return EKX_OK;
+    }
+};
+
+std::string fmt(double v)
+{
+    return std::to_string(v);
+}
+
+ekx_status check_geometry3d(const ekx_geometry3d *g)
+{
+    if (!g) { return fail(EKX_EINVAL, "geometry is NULL"); }
+    // Geometry3D setters (src/geometry3d.cpp:74-261)
+    if (g->nx < 3)
+    {
+        return fail(EKX_EINVAL, "Number of grid points in x = " +
+                                    std::to_string(g->nx) +
+                                    " must be at least 3");
+    }
+    if (g->ny < 3)
+    {
+        return fail(EKX_EINVAL, "Number of grid points in y = " +
+                                    std::to_string(g->ny) +
+                                    " must be at least 3");
+    }
+    if (g->nz < 3)
+    {
+        return fail(EKX_EINVAL, "Number of grid points in z = " +
+                                    std::to_string(g->nz) +
+                                    " must be at least 3");
+    }
+    if (!(g->dx > 0))
+    {
+        return fail(EKX_EINVAL, "Grid spacing in x = " + fmt(g->dx) +
+                                    " must be positive");
+    }
+    if (!(g->dy > 0))
+    {
+        return fail(EKX_EINVAL, "Grid spacing in y = " + fmt(g->dy) +
+                                    " must be positive");
+    }
+    if (!(g->dz > 0))
+    {
+        return fail(EKX_EINVAL, "Grid spacing in z = " + fmt(g->dz) +
+                                    " must be positive");
+    }
+    int64_t n = (int64_t) g->nx*g->ny*g->nz;
+    if (n > (int64_t) std::numeric_limits<int32_t>::max())
+    {
+        return fail(EKX_EINVAL, "a single grid is limited to 2^31-1 nodes");
+    }
+    return EKX_OK;
+}
+
+ekx_status check_options(const ekx_solver_options *o)
+{
+    if (!o) { return fail(EKX_EINVAL, "options is NULL"); }
+    if (o->number_of_sweeps < 0 || o->number_of_sweeps > 65535)
+    {
+        return fail(EKX_EINVAL, "number_of_sweeps must be in [0, 65535]");
+    }
+    if (o->precision < EKX_PRECISION_FP64 || o->precision > EKX_PRECISION_FP32)
+    {
+        return fail(EKX_EINVAL, "unknown precision");
+    }
+    if (o->schedule != EKX_SCHEDULE_COMPLETED &&
+        o->schedule != EKX_SCHEDULE_SHIPPED)
+    {
+        return fail(EKX_EINVAL, "unknown schedule");
+    }
+    if (o->kernel < EKX_KERNEL_AUTO || o->kernel > EKX_KERNEL_COLUMN)
+    {
+        return fail(EKX_EINVAL, "unknown kernel");
+    }
+    if (o->algorithm != EKX_LEVEL_SET_METHOD &&
+        o->algorithm != EKX_FAST_SWEEPING_METHOD)
+    {
+        return fail(EKX_EINVAL, "unknown algorithm");
+    }
+    return EKX_OK;
+}
+
+// Source3D::setLocationIn{X,Y,Z} (src/source3d.cpp:161-291) and the range
+// checks of Solver3D::setSource (src/solver3d.cpp:299-325).
+struct SourceInfo
+{
+    double offset[3];
+    int cell[3];
+    int cellIndex;
+};
+
+ekx_status locate_source3d(const ekx_geometry3d &g, double x, double y,
+                           double z, SourceInfo &s)
+{
+    const double xmax = g.x0 + (g.nx - 1)*g.dx;
+    const double ymax = g.y0 + (g.ny - 1)*g.dy;
+    const double zmax = g.z0 + (g.nz - 1)*g.dz;
+    if (x < g.x0 || x > xmax)
+    {
+        return fail(EKX_EINVAL, "x source position = " + fmt(x) +
+                                    " must be in range [" + fmt(g.x0) + "," +
+                                    fmt(xmax) + "]");
+    }
+    if (y < g.y0 || y > ymax)
+    {
+        return fail(EKX_EINVAL, "y source position = " + fmt(y) +
+                                    " must be in range [" + fmt(g.y0) + "," +
+                                    fmt(ymax) + "]");
+    }
+    if (z < g.z0 || z > zmax)
+    {
+        return fail(EKX_EINVAL, "z source position = " + fmt(z) +
+                                    " must be in range [" + fmt(g.z0) + "," +
+                                    fmt(zmax) + "]");
+    }
+    const int ncx = g.nx - 1, ncy = g.ny - 1, ncz = g.nz - 1;
+    s.offset[0] = x - g.x0;
+    s.offset[1] = y - g.y0;
+    s.offset[2] = z - g.z0;
+    s.cell[0] = std::min(ncx - 1, (int) (s.offset[0]/g.dx));
+    s.cell[1] = std::min(ncy - 1, (int) (s.offset[1]/g.dy));
+    s.cell[2] = std::min(ncz - 1, (int) (s.offset[2]/g.dz));
+    s.cellIndex = (s.cell[2]*ncy + s.cell[1])*ncx + s.cell[0];
+    return EKX_OK;
+}
+
+// Solver3DSweep::initialize uniform test (include/private/solver3d.hpp:35-40)
+bool is_uniform3d(const ekx_geometry3d &g)
+{
+    const double eps100 = std::numeric_limits<double>::epsilon()*100;
+    return std::abs(g.dx - g.dy) < eps100 && std::abs(g.dx - g.dz) < eps100;
+}
+
+NonUniform3D non_uniform_constants(const ekx_geometry3d &g)
+{
+    NonUniform3D c;
+    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    c.dxInv = 1/dx;
+    c.dyInv = 1/dy;
+    c.dzInv = 1/dz;
+    c.dx2Inv = 1/(dx*dx);
+    c.dy2Inv = 1/(dy*dy);
+    c.dz2Inv = 1/(dz*dz);
+    c.dx_dz = dx/dz;
+    c.dz_dx = dz/dx;
+    c.dy_dz = dy/dz;
+    c.dz_dy = dz/dy;
+    c.dx_dy = dx/dy;
+    c.dy_dx = dy/dx;
+    c.dx2_p_dz2_inv = 1/(dx*dx + dz*dz);
+    c.dy2_p_dz2_inv = 1/(dy*dy + dz*dz);
+    c.dx2_p_dy2_inv = 1/(dx*dx + dy*dy);
+    return c;
+}
+
+struct HostOctant
+{
+    int jx0, jy0, jz0;
+};
+
+HostOctant host_octant(const Problem3D &p, int sweep, bool initialize)
+{
+    HostOctant o{1, 1, 1};
+    if (initialize)
+    {
+        int negX = sweep & 1, negY = (sweep >> 1) & 1, negZ = (sweep >> 2) & 1;
+        o.jx0 = negX ? std::max(p.nx - 2 - p.srcCellX, 1)
+                     : std::max(1, p.srcCellX);
+        o.jy0 = negY ? std::max(p.ny - 2 - p.srcCellY, 1)
+                     : std::max(1, p.srcCellY);
+        o.jz0 = negZ ? std::max(p.nz - 2 - p.srcCellZ, 1)
+                     : std::max(1, p.srcCellZ);
+    }
+    return o;
+}
+
+} // namespace
+
+namespace ekx
+{
+void set_last_error(const std::string &message) { g_error = message; }
+} // namespace ekx
+
+// ===========================================================================
+// Engine3D: a batch of 3D problems and the schedule that solves them.
+// ===========================================================================
+struct Engine3D
+{
+    ekx_solver_options opt{};
+    std::vector<ekx_geometry3d> geometry;
+    std::vector<Problem3D> problems;
+    std::vector<SourceInfo> sources;
+    std::vector<bool> haveSource;
+    std::vector<int> modelOf;        // problem -> model
+    std::vector<int64_t> modelOffset;
+    int64_t totalNodes = 0, totalCells = 0, maxNodes = 0;
+    DeviceBuffer<double> S64;
+    DeviceBuffer<float> S32;
+    DeviceBuffer<double> T64;
+    DeviceBuffer<float> T32;
+    DeviceBuffer<Problem3D> dProblems;
+    DeviceBuffer<ColumnItem> dItems;
+    std::vector<int> diagStart;
+    hipStream_t stream = nullptr;
+    hipEvent_t evStart = nullptr, evStop = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    bool graphValid = false;
+    bool haveModel = false;
+    bool haveField = false;
+    bool fieldIsF64 = true;
+    ekx_solve_stats stats{};
+
+    ~Engine3D()
+    {
+        drop_graph();
+        if (evStart) { (void) hipEventDestroy(evStart); }
+        if (evStop) { (void) hipEventDestroy(evStop); }
+        if (stream) { (void) hipStreamDestroy(stream); }
+    }
+
+    void drop_graph()
+    {
+        if (graphExec) { (void) hipGraphExecDestroy(graphExec); }
+        if (graph) { (void) hipGraphDestroy(graph); }
+        graphExec = nullptr;
+        graph = nullptr;
+        graphValid = false;
+    }
+
+    bool sweeps_f64() const
+    {
+        return opt.precision == EKX_PRECISION_FP64;
+    }
+    bool polish() const
+    {
+        return opt.precision == EKX_PRECISION_FP32_POLISH &&
+               opt.schedule == EKX_SCHEDULE_COMPLETED &&
+               opt.number_of_sweeps >= 1;
+    }
+    int kernel() const
+    {
+        return opt.kernel == EKX_KERNEL_AUTO ? EKX_KERNEL_COLUMN : opt.kernel;
+    }
+
+    // Allocates buffers and the column work lists for the given geometries;
+    // modelOf maps every problem to one of nModels models.
+    ekx_status setup(const ekx_solver_options &o,
+                     const std::vector<ekx_geometry3d> &geoms,
+                     const std::vector<int> &models, int nModels)
+    {
+        drop_graph();
+        opt = o;
+        geometry = geoms;
+        modelOf = models;
+        const int n = (int) geoms.size();
+        problems.assign(n, Problem3D{});
+        sources.assign(n, SourceInfo{});
+        haveSource.assign(n, false);
+        haveModel = false;
+        haveField = false;
+        stats = ekx_solve_stats{};
+        if (!stream)
+        {
+            EKX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            EKX_HIP(hipEventCreate(&evStart));
+            EKX_HIP(hipEventCreate(&evStop));
+        }
+        // model offsets (cells)
+        modelOffset.assign(nModels, -1);
+        totalCells = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            int m = modelOf[i];
+            if (modelOffset[m] < 0)
+            {
+                modelOffset[m] = totalCells;
+                const ekx_geometry3d &g = geoms[i];
+                totalCells += (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+            }
+        }
+        totalNodes = 0;
+        maxNodes = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            const ekx_geometry3d &g = geoms[i];
+            Problem3D &p = problems[i];
+            p.nx = g.nx;
+            p.ny = g.ny;
+            p.nz = g.nz;
+            p.radius = o.factored_radius;
+            p.uniform = is_uniform3d(g) ? 1 : 0;
+            p.dx = g.dx;
+            p.dy = g.dy;
+            p.dz = g.dz;
+            p.tOffset = totalNodes;
+            p.sOffset = modelOffset[modelOf[i]];
+            p.nu = non_uniform_constants(g);
+            int64_t nodes = (int64_t) g.nx*g.ny*g.nz;
+            totalNodes += nodes;
+            maxNodes = std::max(maxNodes, nodes);
+        }
+        EKX_CHECK(S64.allocate((size_t) totalCells));
+        if (!sweeps_f64()) { EKX_CHECK(S32.allocate((size_t) totalCells)); }
+        else { S32.release(); }
+        if (sweeps_f64() || polish())
+        {
+            EKX_CHECK(T64.allocate((size_t) totalNodes));
+            EKX_HIP(hipMemsetAsync(T64.ptr, 0, T64.count*sizeof(double),
+                                   stream));
+        }
+        else { T64.release(); }
+        if (!sweeps_f64())
+        {
+            EKX_CHECK(T32.allocate((size_t) totalNodes));
+            EKX_HIP(hipMemsetAsync(T32.ptr, 0, T32.count*sizeof(float),
+                                   stream));
+        }
+        else { T32.release(); }
+        fieldIsF64 = sweeps_f64() || polish();
+        EKX_CHECK(dProblems.allocate((size_t) n));
+        // Column work lists grouped by diagonal bx + by, all problems.
+        std::vector<std::vector<ColumnItem>> perDiag;
+        for (int i = 0; i < n; ++i)
+        {
+            const ekx_geometry3d &g = geoms[i];
+            int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
+            int nby = (g.ny + kColumnTile - 1)/kColumnTile;
+            if ((int) perDiag.size() < nbx + nby - 1)
+            {
+                perDiag.resize(nbx + nby - 1);
+            }
+            for (int by = 0; by < nby; ++by)
+            {
+                for (int bx = 0; bx < nbx; ++bx)
+                {
+                    perDiag[bx + by].push_back(ColumnItem{i, bx, by, 0});
+                }
+            }
+        }
+        std::vector<ColumnItem> flat;
+        diagStart.assign(1, 0);
+        for (auto &d : perDiag)
+        {
+            flat.insert(flat.end(), d.begin(), d.end());
+            diagStart.push_back((int) flat.size());
+        }
+        EKX_CHECK(dItems.allocate(flat.size()));
+        EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
+                               flat.size()*sizeof(ColumnItem),
+                               hipMemcpyHostToDevice, stream));
+        EKX_HIP(hipStreamSynchronize(stream));
+        return EKX_OK;
+    }
+
+    // Uploads one model (cell slowness, double) to its slot.
+    ekx_status upload_model(int model, const double *slowness, int64_t n)
+    {
+        drop_graph();
+        EKX_HIP(hipMemcpyAsync(S64.ptr + modelOffset[model], slowness,
+                               (size_t) n*sizeof(double),
+                               hipMemcpyHostToDevice, stream));
+        return EKX_OK;
+    }
+
+    ekx_status finish_models()
+    {
+        if (S32.ptr)
+        {
+            EKX_HIP(launch_convert(totalCells, S64.ptr, S32.ptr, stream));
+        }
+        EKX_HIP(hipStreamSynchronize(stream));
+        haveModel = true;
+        haveField = false;
+        return EKX_OK;
+    }
+
+    // Fills the source fields of the problem descriptors (the source cell
+    // slowness is read from the device copy of the model).
+    ekx_status prepare_sources()
+    {
+        const int n = (int) problems.size();
+        std::vector<double> sSrc(n, 0.0);
+        for (int i = 0; i < n; ++i)
+        {
+            Problem3D &p = problems[i];
+            const SourceInfo &s = sources[i];
+            EKX_HIP(hipMemcpyAsync(&sSrc[i],
+                                   S64.ptr + p.sOffset + s.cellIndex,
+                                   sizeof(double), hipMemcpyDeviceToHost,
+                                   stream));
+        }
+        EKX_HIP(hipStreamSynchronize(stream));
+        for (int i = 0; i < n; ++i)
+        {
+            Problem3D &p = problems[i];
+            const SourceInfo &s = sources[i];
+            p.srcCellX = s.cell[0];
+            p.srcCellY = s.cell[1];
+            p.srcCellZ = s.cell[2];
+            p.srcOffX = s.offset[0];
+            p.srcOffY = s.offset[1];
+            p.srcOffZ = s.offset[2];
+            p.srcSlowness = sSrc[i];
+        }
+        EKX_HIP(hipMemcpyAsync(dProblems.ptr, problems.data(),
+                               problems.size()*sizeof(Problem3D),
+                               hipMemcpyHostToDevice, stream));
+        return EKX_OK;
+    }
+
+    template <typename TS>
+    ekx_status enqueue_sweep(int sweep, bool initialize, TS *T, const TS *S,
+                             ekx_solve_stats &st)
+    {
+        const int n = (int) problems.size();
+        for (int i = 0; i < n; ++i)
+        {
+            const Problem3D &p = problems[i];
+            HostOctant o = host_octant(p, sweep, initialize);
+            int64_t upd = (int64_t) (p.nx - o.jx0)*(p.ny - o.jy0)*
+                          (p.nz - o.jz0);
+            st.updates += upd;
+            if (sizeof(TS) == 8) { st.updates_fp64 += upd; }
+        }
+        if (kernel() == EKX_KERNEL_HYPERPLANE)
+        {
+            for (int i = 0; i < n; ++i)
+            {
+                const Problem3D &p = problems[i];
+                HostOctant o = host_octant(p, sweep, initialize);
+                const int lastLevel = (p.nx - 1) + (p.ny - 1) + (p.nz - 1);
+                for (int L = o.jx0 + o.jy0 + o.jz0; L <= lastLevel; ++L)
+                {
+                    int kLo = std::max(o.jz0, L - (p.nx - 1) - (p.ny - 1));
+                    int kHi = std::min(p.nz - 1, L - o.jx0 - o.jy0);
+                    if (kLo > kHi) { continue; }
+                    int nPairs = (kHi - kLo + 1)*(p.ny - o.jy0);
+                    EKX_HIP(launch_hyperplane3d<TS>(
+                        dProblems.ptr, i, sweep, initialize ? 1 : 0, L, kLo,
+                        nPairs, T, S, stream));
+                    st.kernel_launches += 1;
+                }
+            }
+        }
+        else
+        {
+            const int nDiag = (int) diagStart.size() - 1;
+            for (int d = 0; d < nDiag; ++d)
+            {
+                int count = diagStart[d + 1] - diagStart[d];
+                EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
+                                             dItems.ptr + diagStart[d], count,
+                                             sweep, initialize ? 1 : 0, T, S,
+                                             stream));
+                st.kernel_launches += 1;
+            }
+        }
+        return EKX_OK;
+    }
+
+    template <typename TS>
+    ekx_status enqueue_phase(TS *T, const TS *S, bool doInit, int iterations,
+                             ekx_solve_stats &st)
+    {
+        if (doInit)
+        {
+            EKX_HIP(launch_init3d<TS>(dProblems.ptr, (int) problems.size(),
+                                      maxNodes, T, stream));
+            if (opt.schedule == EKX_SCHEDULE_SHIPPED)
+            {
+                // src/solver3d.cpp:451-463 (FSM); level set: nothing.
+                if (opt.algorithm == EKX_FAST_SWEEPING_METHOD)
+                {
+                    EKX_CHECK(enqueue_sweep<TS>(0, true, T, S, st));
+                    EKX_CHECK(enqueue_sweep<TS>(1, true, T, S, st));
+                }
+                return EKX_OK;
+            }
+            for (int s = 0; s < 8; ++s)
+            {
+                EKX_CHECK(enqueue_sweep<TS>(s, true, T, S, st));
+            }
+        }
+        for (int k = 0; k < iterations; ++k)
+        {
+            for (int s = 0; s < 8; ++s)
+            {
+                EKX_CHECK(enqueue_sweep<TS>(s, false, T, S, st));
+            }
+            st.iterations_run += 1;
+        }
+        return EKX_OK;
+    }
+
+    // Issues every launch of one solve on the stream.
+    ekx_status enqueue_solve(ekx_solve_stats &st)
+    {
+        st = ekx_solve_stats{};
+        const int nIt = opt.number_of_sweeps;
+        if (sweeps_f64())
+        {
+            EKX_CHECK(enqueue_phase<double>(T64.ptr, S64.ptr, true, nIt, st));
+        }
+        else
+        {
+            int itF32 = polish() ? nIt - 1 : nIt;
+            if (opt.schedule == EKX_SCHEDULE_SHIPPED) { itF32 = 0; }
+            EKX_CHECK(enqueue_phase<float>(T32.ptr, S32.ptr, true, itF32, st));
+            if (polish())
+            {
+                EKX_HIP(launch_convert(totalNodes, T32.ptr, T64.ptr, stream));
+                EKX_CHECK(enqueue_phase<double>(T64.ptr, S64.ptr, false, 1,
+                                                st));
+            }
+        }
+        return EKX_OK;
+    }
+
+    ekx_status solve()
+    {
+        if (!haveModel)
+        {
+            return fail(EKX_ESTATE, "Velocity model not set");
+        }
+        for (size_t i = 0; i < problems.size(); ++i)
+        {
+            if (!haveSource[i])
+            {
+                return fail(EKX_ESTATE, "Source location not set");
+            }
+        }
+        haveField = false;
+        EKX_CHECK(prepare_sources());
+        ekx_solve_stats st{};
+        if (opt.use_graph)
+        {
+            if (!graphValid)
+            {
+                drop_graph();
+                EKX_HIP(hipStreamBeginCapture(stream,
+                                              hipStreamCaptureModeThreadLocal));
+                ekx_status s = enqueue_solve(st);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(stream, &g);
+                if (s != EKX_OK)
+                {
+                    if (g) { (void) hipGraphDestroy(g); }
+                    return s;
+                }
+                EKX_HIP(e);
+                graph = g;
+                EKX_HIP(hipGraphInstantiate(&graphExec, graph, nullptr,
+                                            nullptr, 0));
+                graphValid = true;
+                stats = st;
+            }
+            else
+            {
+                st = stats;
+            }
+            EKX_HIP(hipEventRecord(evStart, stream));
+            EKX_HIP(hipGraphLaunch(graphExec, stream));
+            EKX_HIP(hipEventRecord(evStop, stream));
+        }
+        else
+        {
+            EKX_HIP(hipEventRecord(evStart, stream));
+            EKX_CHECK(enqueue_solve(st));
+            EKX_HIP(hipEventRecord(evStop, stream));
+        }
+        EKX_HIP(hipEventSynchronize(evStop));
+        EKX_HIP(hipGetLastError());
+        float ms = 0;
+        EKX_HIP(hipEventElapsedTime(&ms, evStart, evStop));
+        st.last_solve_ms = ms;
+        stats = st;
+        haveField = true;
+        return EKX_OK;
+    }
+
+    ekx_status copy_field(int problem, int64_t n, double *out) const
+    {
+        const ekx_geometry3d &g = geometry[problem];
+        int64_t nGrid = (int64_t) g.nx*g.ny*g.nz;
+        if (n != nGrid)
+        {
+            return fail(EKX_EINVAL, "output length " + std::to_string(n) +
+                                        " must equal the number of grid "
+                                        "points " + std::to_string(nGrid));
+        }
+        if (!out) { return fail(EKX_EINVAL, "output is NULL"); }
+        const int64_t off = problems[problem].tOffset;
+        if (fieldIsF64)
+        {
+            EKX_HIP(hipMemcpy(out, T64.ptr + off, (size_t) n*sizeof(double),
+                              hipMemcpyDeviceToHost));
+        }
+        else
+        {
+            std::vector<float> tmp((size_t) n);
+            EKX_HIP(hipMemcpy(tmp.data(), T32.ptr + off,
+                              (size_t) n*sizeof(float),
+                              hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) { out[i] = tmp[(size_t) i]; }
+        }
+        return EKX_OK;
+    }
+};
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+struct ekx_solver3d
+{
+    Engine3D engine;
+    ekx_geometry3d geometry{};
+    bool initialized = false;
+};
+
+struct ekx_batch3d
+{
+    Engine3D engine;
+    bool initialized = false;
+};
+
+extern "C" {
+
+const char *ekx_last_error(void) { return g_error.c_str(); }
+
+const char *ekx_version(void) { return "0.1.0"; }
+
+void ekx_solver_options_default(ekx_solver_options *o)
+{
+    if (!o) { return; }
+    o->number_of_sweeps = 5;
+    o->factored_radius = 3;
+    o->algorithm = EKX_LEVEL_SET_METHOD;
+    o->verbosity = 0;
+    o->convergence_tolerance = 1.e-5;
+    o->precision = EKX_PRECISION_FP64;
+    o->schedule = EKX_SCHEDULE_COMPLETED;
+    o->kernel = EKX_KERNEL_AUTO;
+    o->early_exit = 0;
+    o->use_graph = 1;
+}
+
+ekx_status ekx_device_count(int *count)
+{
+    if (!count) { return fail(EKX_EINVAL, "count is NULL"); }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { n = 0; }
+    *count = n;
+    return EKX_OK;
+}
+
+ekx_status ekx_device_synchronize(void)
+{
+    EKX_HIP(hipDeviceSynchronize());
+    return EKX_OK;
+}
+
+// ----------------------------- Solver3D -----------------------------------
+ekx_status ekx_solver3d_create(ekx_solver3d **solver)
+{
+    if (!solver) { return fail(EKX_EINVAL, "solver is NULL"); }
+    *solver = new (std::nothrow) ekx_solver3d();
+    if (!*solver) { return fail(EKX_ENOMEM, "out of host memory"); }
+    return EKX_OK;
+}
+
+void ekx_solver3d_destroy(ekx_solver3d *solver) { delete solver; }
+
+ekx_status ekx_solver3d_initialize(ekx_solver3d *h,
+                                   const ekx_solver_options *options,
+                                   const ekx_geometry3d *geometry)
+{
+    if (!h) { return fail(EKX_EINVAL, "solver is NULL"); }
+    h->initialized = false;
+    EKX_CHECK(check_options(options));
+    EKX_CHECK(check_geometry3d(geometry));
+    h->geometry = *geometry;
+    EKX_CHECK(h->engine.setup(*options, {*geometry}, {0}, 1));
+    h->initialized = true;
+    return EKX_OK;
+}
+
+int ekx_solver3d_is_initialized(const ekx_solver3d *h)
+{
+    return h && h->initialized ? 1 : 0;
+}
+
+static ekx_status require_initialized(const ekx_solver3d *h)
+{
+    if (!h) { return fail(EKX_EINVAL, "solver is NULL"); }
+    if (!h->initialized) { return fail(EKX_ESTATE, "Class not initialized"); }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_set_slowness(ekx_solver3d *h, int64_t n,
+                                     const double *slowness)
+{
+    EKX_CHECK(require_initialized(h));
+    h->engine.haveModel = false;
+    h->engine.haveField = false;
+    const ekx_geometry3d &g = h->geometry;
+    int64_t nCell = (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+    if (n != nCell)
+    {
+        return fail(EKX_EINVAL, "Number of cells = " + std::to_string(n) +
+                                    " must equal " + std::to_string(nCell));
+    }
+    if (!slowness) { return fail(EKX_EINVAL, "slowness is NULL"); }
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (!(slowness[i] > 0))
+        {
+            return fail(EKX_EINVAL, "Non-positive slowness detected");
+        }
+    }
+    EKX_CHECK(h->engine.upload_model(0, slowness, n));
+    return h->engine.finish_models();
+}
+
+ekx_status ekx_solver3d_set_nodal_velocities(ekx_solver3d *h, int64_t n,
+                                             const double *v, int ordering)
+{
+    EKX_CHECK(require_initialized(h));
+    h->engine.haveModel = false;
+    h->engine.haveField = false;
+    const ekx_geometry3d &g = h->geometry;
+    int64_t nGrid = (int64_t) g.nx*g.ny*g.nz;
+    if (n != nGrid)
+    {
+        return fail(EKX_EINVAL, "Number of grid points = " +
+                                    std::to_string(n) + " must equal " +
+                                    std::to_string(nGrid));
+    }
+    if (!v) { return fail(EKX_EINVAL, "velocities is NULL"); }
+    if (ordering != EKX_ORDERING_NATURAL && ordering != EKX_ORDERING_XYZ)
+    {
+        return fail(EKX_EINVAL, "unknown ordering");
+    }
+    // Model3D::setNodalVelocities rejects v <= 0 (src/model3d.cpp:303-307)
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (!(v[i] > 0))
+        {
+            return fail(EKX_EINVAL, "Non-positive velocities detected");
+        }
+    }
+    Engine3D &e = h->engine;
+    e.drop_graph();
+    DeviceBuffer<double> dv;
+    EKX_CHECK(dv.allocate((size_t) n));
+    EKX_HIP(hipMemcpyAsync(dv.ptr, v, (size_t) n*sizeof(double),
+                           hipMemcpyHostToDevice, e.stream));
+    EKX_HIP(launch_nodal_to_cell3d(g.nx, g.ny, g.nz, dv.ptr, ordering,
+                                   e.S64.ptr, e.stream));
+    EKX_HIP(hipStreamSynchronize(e.stream));
+    return e.finish_models();
+}
+
+ekx_status ekx_solver3d_set_cell_velocities(ekx_solver3d *h, int64_t n,
+                                            const double *v, int ordering)
+{
+    EKX_CHECK(require_initialized(h));
+    h->engine.haveModel = false;
+    h->engine.haveField = false;
+    const ekx_geometry3d &g = h->geometry;
+    int64_t nCell = (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+    if (n != nCell)
+    {
+        return fail(EKX_EINVAL, "Number of cells = " + std::to_string(n) +
+                                    " must equal " + std::to_string(nCell));
+    }
+    if (!v) { return fail(EKX_EINVAL, "velocities is NULL"); }
+    if (ordering != EKX_ORDERING_NATURAL && ordering != EKX_ORDERING_XYZ)
+    {
+        return fail(EKX_EINVAL, "unknown ordering");
+    }
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (!(v[i] > 0))
+        {
+            return fail(EKX_EINVAL, "Non-positive velocities detected");
+        }
+    }
+    Engine3D &e = h->engine;
+    e.drop_graph();
+    DeviceBuffer<double> dv;
+    EKX_CHECK(dv.allocate((size_t) n));
+    EKX_HIP(hipMemcpyAsync(dv.ptr, v, (size_t) n*sizeof(double),
+                           hipMemcpyHostToDevice, e.stream));
+    EKX_HIP(launch_cell_velocity_to_slowness3d(g.nx, g.ny, g.nz, dv.ptr,
+                                               ordering, e.S64.ptr,
+                                               e.stream));
+    EKX_HIP(hipStreamSynchronize(e.stream));
+    return e.finish_models();
+}
+
+int ekx_solver3d_have_velocity_model(const ekx_solver3d *h)
+{
+    return h && h->initialized && h->engine.haveModel ? 1 : 0;
+}
+
+ekx_status ekx_solver3d_set_source(ekx_solver3d *h, double x, double y,
+                                   double z)
+{
+    EKX_CHECK(require_initialized(h));
+    h->engine.haveSource[0] = false;
+    h->engine.haveField = false;
+    SourceInfo s;
+    EKX_CHECK(locate_source3d(h->geometry, x, y, z, s));
+    h->engine.sources[0] = s;
+    h->engine.haveSource[0] = true;
+    return EKX_OK;
+}
+
+int ekx_solver3d_have_source(const ekx_solver3d *h)
+{
+    return h && h->initialized && h->engine.haveSource[0] ? 1 : 0;
+}
+
+ekx_status ekx_solver3d_get_source_cell(const ekx_solver3d *h,
+                                        double offset[3], int cell[3],
+                                        int *cell_index)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!h->engine.haveSource[0])
+    {
+        return fail(EKX_ESTATE, "Source location not set");
+    }
+    const SourceInfo &s = h->engine.sources[0];
+    for (int k = 0; k < 3; ++k)
+    {
+        if (offset) { offset[k] = s.offset[k]; }
+        if (cell) { cell[k] = s.cell[k]; }
+    }
+    if (cell_index) { *cell_index = s.cellIndex; }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_solve(ekx_solver3d *h)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.solve();
+}
+
+int ekx_solver3d_have_travel_time_field(const ekx_solver3d *h)
+{
+    return h && h->initialized && h->engine.haveField ? 1 : 0;
+}
+
+ekx_status ekx_solver3d_get_travel_times(const ekx_solver3d *h, int64_t n,
+                                         double *out)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.copy_field(0, n, out);
+}
+
+ekx_status ekx_solver3d_get_travel_times_f32(const ekx_solver3d *h,
+                                             int64_t n, float *out)
+{
+    EKX_CHECK(require_initialized(h));
+    const Engine3D &e = h->engine;
+    const ekx_geometry3d &g = h->geometry;
+    int64_t nGrid = (int64_t) g.nx*g.ny*g.nz;
+    if (n != nGrid || !out)
+    {
+        return fail(EKX_EINVAL, "output must hold the " +
+                                    std::to_string(nGrid) + " grid points");
+    }
+    if (e.T32.ptr && !e.fieldIsF64)
+    {
+        EKX_HIP(hipMemcpy(out, e.T32.ptr, (size_t) n*sizeof(float),
+                          hipMemcpyDeviceToHost));
+        return EKX_OK;
+    }
+    std::vector<double> tmp((size_t) n);
+    EKX_CHECK(e.copy_field(0, n, tmp.data()));
+    for (int64_t i = 0; i < n; ++i) { out[i] = (float) tmp[(size_t) i]; }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_get_travel_times_device(const ekx_solver3d *h,
+                                                const void **ptr,
+                                                int *element_size)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!h->engine.haveField)
+    {
+        return fail(EKX_ESTATE, "Travel time field not yet computed");
+    }
+    const Engine3D &e = h->engine;
+    if (ptr)
+    {
+        *ptr = e.fieldIsF64 ? (const void *) e.T64.ptr
+                            : (const void *) e.T32.ptr;
+    }
+    if (element_size) { *element_size = e.fieldIsF64 ? 8 : 4; }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_get_stats(const ekx_solver3d *h,
+                                  ekx_solve_stats *stats)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!stats) { return fail(EKX_EINVAL, "stats is NULL"); }
+    *stats = h->engine.stats;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_get_stream(const ekx_solver3d *h, void **stream)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!stream) { return fail(EKX_EINVAL, "stream is NULL"); }
+    *stream = (void *) h->engine.stream;
+    return EKX_OK;
+}
+
+// ------------------------------ Batch3D -----------------------------------
+ekx_status ekx_batch3d_create(ekx_batch3d **batch)
+{
+    if (!batch) { return fail(EKX_EINVAL, "batch is NULL"); }
+    *batch = new (std::nothrow) ekx_batch3d();
+    if (!*batch) { return fail(EKX_ENOMEM, "out of host memory"); }
+    return EKX_OK;
+}
+
+void ekx_batch3d_destroy(ekx_batch3d *batch) { delete batch; }
+
+ekx_status ekx_batch3d_initialize(ekx_batch3d *h,
+                                  const ekx_solver_options *options,
+                                  int n, const ekx_problem3d *problems)
+{
+    if (!h) { return fail(EKX_EINVAL, "batch is NULL"); }
+    h->initialized = false;
+    EKX_CHECK(check_options(options));
+    if (n < 1 || !problems)
+    {
+        return fail(EKX_EINVAL, "a batch needs at least one problem");
+    }
+    std::vector<ekx_geometry3d> geoms((size_t) n);
+    std::vector<int> modelOf((size_t) n);
+    std::map<std::tuple<const double *, int, int, int>, int> models;
+    std::vector<const double *> modelPtr;
+    for (int i = 0; i < n; ++i)
+    {
+        const ekx_problem3d &pr = problems[i];
+        EKX_CHECK(check_geometry3d(&pr.geometry));
+        if (!pr.slowness)
+        {
+            return fail(EKX_EINVAL, "problem " + std::to_string(i) +
+                                        ": slowness is NULL");
+        }
+        geoms[i] = pr.geometry;
+        auto key = std::make_tuple(pr.slowness, pr.geometry.nx,
+                                   pr.geometry.ny, pr.geometry.nz);
+        auto it = models.find(key);
+        if (it == models.end())
+        {
+            int m = (int) modelPtr.size();
+            models.emplace(key, m);
+            modelPtr.push_back(pr.slowness);
+            modelOf[i] = m;
+        }
+        else { modelOf[i] = it->second; }
+    }
+    Engine3D &e = h->engine;
+    EKX_CHECK(e.setup(*options, geoms, modelOf, (int) modelPtr.size()));
+    std::vector<bool> uploaded(modelPtr.size(), false);
+    for (int i = 0; i < n; ++i)
+    {
+        int m = modelOf[i];
+        if (uploaded[m]) { continue; }
+        const ekx_geometry3d &g = geoms[i];
+        int64_t nCell = (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+        const double *s = modelPtr[m];
+        for (int64_t k = 0; k < nCell; ++k)
+        {
+            if (!(s[k] > 0))
+            {
+                return fail(EKX_EINVAL, "problem " + std::to_string(i) +
+                                            ": non-positive slowness");
+            }
+        }
+        EKX_CHECK(e.upload_model(m, s, nCell));
+        uploaded[m] = true;
+    }
+    EKX_CHECK(e.finish_models());
+    for (int i = 0; i < n; ++i)
+    {
+        SourceInfo s;
+        EKX_CHECK(locate_source3d(geoms[i], problems[i].source[0],
+                                  problems[i].source[1],
+                                  problems[i].source[2], s));
+        e.sources[i] = s;
+        e.haveSource[i] = true;
+    }
+    h->initialized = true;
+    return EKX_OK;
+}
+
+static ekx_status require_batch(const ekx_batch3d *h)
+{
+    if (!h) { return fail(EKX_EINVAL, "batch is NULL"); }
+    if (!h->initialized) { return fail(EKX_ESTATE, "Class not initialized"); }
+    return EKX_OK;
+}
+
+ekx_status ekx_batch3d_set_source(ekx_batch3d *h, int problem, double x,
+                                  double y, double z)
+{
+    EKX_CHECK(require_batch(h));
+    Engine3D &e = h->engine;
+    if (problem < 0 || problem >= (int) e.problems.size())
+    {
+        return fail(EKX_EINVAL, "problem index out of range");
+    }
+    e.haveField = false;
+    e.haveSource[problem] = false;
+    SourceInfo s;
+    EKX_CHECK(locate_source3d(e.geometry[problem], x, y, z, s));
+    e.sources[problem] = s;
+    e.haveSource[problem] = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_batch3d_solve(ekx_batch3d *h)
+{
+    EKX_CHECK(require_batch(h));
+    return h->engine.solve();
+}
+
+ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *h, int problem,
+                                        int64_t n, double *out)
+{
+    EKX_CHECK(require_batch(h));
+    if (problem < 0 || problem >= (int) h->engine.problems.size())
+    {
+        return fail(EKX_EINVAL, "problem index out of range");
+    }
+    return h->engine.copy_field(problem, n, out);
+}
+
+ekx_status ekx_batch3d_get_stats(const ekx_batch3d *h,
+                                 ekx_solve_stats *stats)
+{
+    EKX_CHECK(require_batch(h));
+    if (!stats) { return fail(EKX_EINVAL, "stats is NULL"); }
+    *stats = h->engine.stats;
+    return EKX_OK;
+}
+
+ekx_status ekx_batch3d_get_stream(const ekx_batch3d *h, void **stream)
+{
+    EKX_CHECK(require_batch(h));
+    if (!stream) { return fail(EKX_EINVAL, "stream is NULL"); }
+    *stream = (void *) h->engine.stream;
+    return EKX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,447 @@
+// ekx_capi2d.hip -- C ABI of the 2D solver (include/eikonalxx_amd.h).
+#include "eikonalxx_amd.h"
+#include "ekx_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <string>
+#include <vector>
+
+using namespace ekx;
+
+// The 3D translation unit owns the per-thread error message.
+namespace ekx
+{
+void set_last_error(const std::string &message);
+}
+
+namespace
+{
+
+ekx_status fail2(ekx_status status, const std::string &message)
+{
+    ekx::set_last_error(message);
+    return status;
+}
+
+#define EKX2_HIP(expr)                                                       \
+    do                                                                       \
+    {                                                                        \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess)                                                \
+        {                                                                    \
+            return fail2(EKX_EHIP, std::string(#expr) + ": " +               \
+                                       hipGetErrorString(e_));               \
+        }                                                                    \
+    } while (0)
+
+#define EKX2_CHECK(expr)                                                     \
+    do                                                                       \
+    {                                                                        \
+        ekx_status s_ = (expr);                                              \
+        if (s_ != EKX_OK) { return s_; }                                     \
+    } while (0)
+
+template <typename T> struct Buf
+{
+    T *ptr = nullptr;
+    size_t count = 0;
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    ~Buf() { release(); }
+    void release()
+    {
+        if (ptr) { (void) hipFree(ptr); }
+        ptr = nullptr;
+        count = 0;
+    }
+    ekx_status allocate(size_t n)
+    {
+        if (n == count && ptr) { return EKX_OK; }
+        release();
+        if (n == 0) { return EKX_OK; }
+        hipError_t e = hipMalloc((void **) &ptr, n*sizeof(T));
+        if (e != hipSuccess)
+        {
+            ptr = nullptr;
+            return fail2(EKX_ENOMEM, std::string("hipMalloc failed: ") +
+                                         hipGetErrorString(e));
+        }
+        count = n;
+        return EKX_OK;
+    }
+};
+
+std::string f2s(double v) { return std::to_string(v); }
+
+struct Loc2D
+{
+    double off[2];
+    int cell[2];
+};
+
+// Source2D / Station2D location -> offset and cell (src/source2d.cpp,
+// src/station2d.cpp:155-158, 210-213).
+ekx_status locate2d(const ekx_geometry2d &g, double x, double z, Loc2D &l,
+                    const char *what)
+{
+    const double xmax = g.x0 + (g.nx - 1)*g.dx;
+    const double zmax = g.z0 + (g.nz - 1)*g.dz;
+    if (x < g.x0 || x > xmax)
+    {
+        return fail2(EKX_EINVAL, std::string("x ") + what + " position = " +
+                                     f2s(x) + " must be in range [" +
+                                     f2s(g.x0) + "," + f2s(xmax) + "]");
+    }
+    if (z < g.z0 || z > zmax)
+    {
+        return fail2(EKX_EINVAL, std::string("z ") + what + " position = " +
+                                     f2s(z) + " must be in range [" +
+                                     f2s(g.z0) + "," + f2s(zmax) + "]");
+    }
+    l.off[0] = x - g.x0;
+    l.off[1] = z - g.z0;
+    l.cell[0] = std::min(g.nx - 2, (int) (l.off[0]/g.dx));
+    l.cell[1] = std::min(g.nz - 2, (int) (l.off[1]/g.dz));
+    return EKX_OK;
+}
+
+} // namespace
+
+struct ekx_solver2d
+{
+    ekx_solver_options opt{};
+    ekx_geometry2d geometry{};
+    Problem2D problem{};
+    Loc2D source{};
+    std::vector<Loc2D> stations;
+    Buf<double> S, T, stationOut, stationX, stationZ;
+    Buf<int> stationCX, stationCZ;
+    Buf<Problem2D> dProblem;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool initialized = false, haveModel = false, haveSource = false;
+    bool haveField = false;
+    std::vector<double> stationTimes;
+    ekx_solve_stats stats{};
+    ~ekx_solver2d()
+    {
+        if (ev0) { (void) hipEventDestroy(ev0); }
+        if (ev1) { (void) hipEventDestroy(ev1); }
+        if (stream) { (void) hipStreamDestroy(stream); }
+    }
+};
+
+static ekx_status require2d(const ekx_solver2d *h)
+{
+    if (!h) { return fail2(EKX_EINVAL, "solver is NULL"); }
+    if (!h->initialized)
+    {
+        return fail2(EKX_ESTATE, "Class not initialized");
+    }
+    return EKX_OK;
+}
+
+extern "C" {
+
+ekx_status ekx_solver2d_create(ekx_solver2d **solver)
+{
+    if (!solver) { return fail2(EKX_EINVAL, "solver is NULL"); }
+    *solver = new (std::nothrow) ekx_solver2d();
+    if (!*solver) { return fail2(EKX_ENOMEM, "out of host memory"); }
+    return EKX_OK;
+}
+
+void ekx_solver2d_destroy(ekx_solver2d *solver) { delete solver; }
+
+ekx_status ekx_solver2d_initialize(ekx_solver2d *h,
+                                   const ekx_solver_options *options,
+                                   const ekx_geometry2d *g)
+{
+    if (!h) { return fail2(EKX_EINVAL, "solver is NULL"); }
+    h->initialized = false;
+    h->haveModel = h->haveSource = h->haveField = false;
+    if (!options || !g)
+    {
+        return fail2(EKX_EINVAL, "options and geometry must not be NULL");
+    }
+    if (options->number_of_sweeps < 0 || options->number_of_sweeps > 65535)
+    {
+        return fail2(EKX_EINVAL, "number_of_sweeps must be in [0, 65535]");
+    }
+    // Geometry2D setters (src/geometry2d.cpp)
+    if (g->nx < 3 || g->nz < 3)
+    {
+        return fail2(EKX_EINVAL, "Number of grid points must be at least 3");
+    }
+    if (!(g->dx > 0) || !(g->dz > 0))
+    {
+        return fail2(EKX_EINVAL, "Grid spacing must be positive");
+    }
+    if ((int64_t) g->nx*g->nz > (int64_t) std::numeric_limits<int>::max())
+    {
+        return fail2(EKX_EINVAL, "grid too large");
+    }
+    h->opt = *options;
+    h->geometry = *g;
+    if (!h->stream)
+    {
+        EKX2_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        EKX2_HIP(hipEventCreate(&h->ev0));
+        EKX2_HIP(hipEventCreate(&h->ev1));
+    }
+    Problem2D &p = h->problem;
+    p = Problem2D{};
+    p.nx = g->nx;
+    p.nz = g->nz;
+    p.radius = options->factored_radius;
+    p.nSweeps = options->number_of_sweeps;
+    // Solver2DSweep::initialize (include/private/solver2d.hpp:29-46)
+    p.uniform = std::abs(g->dx - g->dz) <
+                std::numeric_limits<double>::epsilon()*100 ? 1 : 0;
+    p.dx = g->dx;
+    p.dz = g->dz;
+    p.dx_dz = g->dx/g->dz;
+    p.dz_dx = g->dz/g->dx;
+    p.cosTheta = g->dx/std::hypot(g->dx, g->dz);
+    p.sinTheta = g->dz/std::hypot(g->dx, g->dz);
+    p.tOffset = 0;
+    p.sOffset = 0;
+    EKX2_CHECK(h->S.allocate((size_t) (g->nx - 1)*(g->nz - 1)));
+    EKX2_CHECK(h->T.allocate((size_t) g->nx*g->nz));
+    EKX2_CHECK(h->dProblem.allocate(1));
+    EKX2_HIP(hipMemsetAsync(h->T.ptr, 0, h->T.count*sizeof(double),
+                            h->stream));
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    h->stations.clear();
+    h->stationTimes.clear();
+    h->initialized = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_set_slowness(ekx_solver2d *h, int64_t n,
+                                     const double *s)
+{
+    EKX2_CHECK(require2d(h));
+    h->haveModel = false;
+    h->haveField = false;
+    int64_t nCell = (int64_t) (h->geometry.nx - 1)*(h->geometry.nz - 1);
+    if (n != nCell || !s)
+    {
+        return fail2(EKX_EINVAL, "Number of cells = " + std::to_string(n) +
+                                     " must equal " + std::to_string(nCell));
+    }
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (!(s[i] > 0))
+        {
+            return fail2(EKX_EINVAL, "Non-positive slowness detected");
+        }
+    }
+    EKX2_HIP(hipMemcpyAsync(h->S.ptr, s, (size_t) n*sizeof(double),
+                            hipMemcpyHostToDevice, h->stream));
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    h->haveModel = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_set_nodal_velocities(ekx_solver2d *h, int64_t n,
+                                             const double *v, int ordering)
+{
+    EKX2_CHECK(require2d(h));
+    h->haveModel = false;
+    h->haveField = false;
+    int64_t nGrid = (int64_t) h->geometry.nx*h->geometry.nz;
+    if (n != nGrid || !v)
+    {
+        return fail2(EKX_EINVAL, "Number of grid points = " +
+                                     std::to_string(n) + " must equal " +
+                                     std::to_string(nGrid));
+    }
+    if (ordering != EKX_ORDERING_NATURAL && ordering != EKX_ORDERING_XYZ)
+    {
+        return fail2(EKX_EINVAL, "unknown ordering");
+    }
+    for (int64_t i = 0; i < n; ++i)
+    {
+        if (!(v[i] > 0))
+        {
+            return fail2(EKX_EINVAL, "Non-positive velocities detected");
+        }
+    }
+    Buf<double> dv;
+    EKX2_CHECK(dv.allocate((size_t) n));
+    EKX2_HIP(hipMemcpyAsync(dv.ptr, v, (size_t) n*sizeof(double),
+                            hipMemcpyHostToDevice, h->stream));
+    EKX2_HIP(launch_nodal_to_cell2d(h->geometry.nx, h->geometry.nz, dv.ptr,
+                                    ordering, h->S.ptr, h->stream));
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    h->haveModel = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_set_source(ekx_solver2d *h, double x, double z)
+{
+    EKX2_CHECK(require2d(h));
+    h->haveSource = false;
+    h->haveField = false;
+    Loc2D l;
+    EKX2_CHECK(locate2d(h->geometry, x, z, l, "source"));
+    h->source = l;
+    h->haveSource = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_set_stations(ekx_solver2d *h, int n, const double *x,
+                                     const double *z)
+{
+    EKX2_CHECK(require2d(h));
+    if (n < 0 || (n > 0 && (!x || !z)))
+    {
+        return fail2(EKX_EINVAL, "invalid station arrays");
+    }
+    std::vector<Loc2D> locs((size_t) n);
+    for (int i = 0; i < n; ++i)
+    {
+        EKX2_CHECK(locate2d(h->geometry, x[i], z[i], locs[i], "station"));
+    }
+    h->stations = locs;
+    h->stationTimes.clear();
+    std::vector<double> xo(n), zo(n);
+    std::vector<int> cx(n), cz(n);
+    for (int i = 0; i < n; ++i)
+    {
+        xo[i] = locs[i].off[0];
+        zo[i] = locs[i].off[1];
+        cx[i] = locs[i].cell[0];
+        cz[i] = locs[i].cell[1];
+    }
+    size_t m = (size_t) std::max(n, 1);
+    EKX2_CHECK(h->stationX.allocate(m));
+    EKX2_CHECK(h->stationZ.allocate(m));
+    EKX2_CHECK(h->stationCX.allocate(m));
+    EKX2_CHECK(h->stationCZ.allocate(m));
+    EKX2_CHECK(h->stationOut.allocate(m));
+    if (n > 0)
+    {
+        EKX2_HIP(hipMemcpy(h->stationX.ptr, xo.data(), n*sizeof(double),
+                           hipMemcpyHostToDevice));
+        EKX2_HIP(hipMemcpy(h->stationZ.ptr, zo.data(), n*sizeof(double),
+                           hipMemcpyHostToDevice));
+        EKX2_HIP(hipMemcpy(h->stationCX.ptr, cx.data(), n*sizeof(int),
+                           hipMemcpyHostToDevice));
+        EKX2_HIP(hipMemcpy(h->stationCZ.ptr, cz.data(), n*sizeof(int),
+                           hipMemcpyHostToDevice));
+    }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_solve(ekx_solver2d *h)
+{
+    EKX2_CHECK(require2d(h));
+    h->haveField = false;
+    if (!h->haveModel)
+    {
+        return fail2(EKX_ESTATE, "Velocity model not set");
+    }
+    if (!h->haveSource)
+    {
+        return fail2(EKX_ESTATE, "Source location not set");
+    }
+    Problem2D &p = h->problem;
+    p.srcCellX = h->source.cell[0];
+    p.srcCellZ = h->source.cell[1];
+    p.srcOffX = h->source.off[0];
+    p.srcOffZ = h->source.off[1];
+    const int ncx = h->geometry.nx - 1;
+    EKX2_HIP(hipMemcpy(&p.srcSlowness,
+                       h->S.ptr + (size_t) p.srcCellZ*ncx + p.srcCellX,
+                       sizeof(double), hipMemcpyDeviceToHost));
+    EKX2_HIP(hipMemcpyAsync(h->dProblem.ptr, &p, sizeof(Problem2D),
+                            hipMemcpyHostToDevice, h->stream));
+    EKX2_HIP(hipEventRecord(h->ev0, h->stream));
+    EKX2_HIP(launch_solve2d(h->dProblem.ptr, 1, h->T.ptr, h->S.ptr,
+                            h->stream));
+    EKX2_HIP(hipEventRecord(h->ev1, h->stream));
+    const int ns = (int) h->stations.size();
+    if (ns > 0)
+    {
+        EKX2_HIP(launch_stations2d(h->dProblem.ptr, ns, h->stationX.ptr,
+                                   h->stationZ.ptr, h->stationCX.ptr,
+                                   h->stationCZ.ptr, h->T.ptr,
+                                   h->stationOut.ptr, h->stream));
+    }
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    EKX2_HIP(hipGetLastError());
+    h->stationTimes.assign((size_t) ns, 0.0);
+    if (ns > 0)
+    {
+        EKX2_HIP(hipMemcpy(h->stationTimes.data(), h->stationOut.ptr,
+                           ns*sizeof(double), hipMemcpyDeviceToHost));
+    }
+    float ms = 0;
+    EKX2_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    ekx_solve_stats st{};
+    // updates: 4 init sweeps over the source quadrants + nSweeps x 4
+    for (int s = 0; s < 4; ++s)
+    {
+        int negX = s & 1, negZ = (s >> 1) & 1;
+        int jx0 = negX ? std::max(p.nx - 2 - p.srcCellX, 1)
+                       : std::max(p.srcCellX, 1);
+        int jz0 = negZ ? std::max(p.nz - 2 - p.srcCellZ, 1)
+                       : std::max(p.srcCellZ, 1);
+        st.updates += (int64_t) (p.nx - jx0)*(p.nz - jz0);
+    }
+    st.updates += (int64_t) p.nSweeps*4*(p.nx - 1)*(p.nz - 1);
+    st.updates_fp64 = st.updates;
+    st.kernel_launches = 1;
+    st.iterations_run = p.nSweeps;
+    st.last_solve_ms = ms;
+    h->stats = st;
+    h->haveField = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_get_travel_times(const ekx_solver2d *h, int64_t n,
+                                         double *out)
+{
+    EKX2_CHECK(require2d(h));
+    int64_t nGrid = (int64_t) h->geometry.nx*h->geometry.nz;
+    if (n != nGrid || !out)
+    {
+        return fail2(EKX_EINVAL, "output must hold the " +
+                                     std::to_string(nGrid) + " grid points");
+    }
+    EKX2_HIP(hipMemcpy(out, h->T.ptr, (size_t) n*sizeof(double),
+                       hipMemcpyDeviceToHost));
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_get_travel_times_at_stations(const ekx_solver2d *h,
+                                                     int n, double *out)
+{
+    EKX2_CHECK(require2d(h));
+    if (!h->haveField)
+    {
+        return fail2(EKX_ESTATE, "Travel time field not yet computed");
+    }
+    if (n != (int) h->stationTimes.size() || (n > 0 && !out))
+    {
+        return fail2(EKX_EINVAL, "output must hold one value per station");
+    }
+    for (int i = 0; i < n; ++i) { out[i] = h->stationTimes[(size_t) i]; }
+    return EKX_OK;
+}
+
+ekx_status ekx_solver2d_get_stats(const ekx_solver2d *h,
+                                  ekx_solve_stats *stats)
+{
+    EKX2_CHECK(require2d(h));
+    if (!stats) { return fail2(EKX_EINVAL, "stats is NULL"); }
+    *stats = h->stats;
+    return EKX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,430 @@
+// ekx_kernels2d.hip -- the 2D fast-sweeping solver on the device (gfx950).
+//
+// The schedule is Solver2D::solve with the fast-sweeping method
+// (src/solver2d.cpp:462-565): source initialisation, the four
+// initialization sweeps, then nSweeps x four refinement sweeps.  Each sweep
+// runs in anti-diagonal order (levels jx + jz = L in the sweep's permuted
+// frame, solverUtilities2d.hpp:1065-1147); a node reads only neighbours on
+// earlier levels, so the result equals the serial loop of
+// Solver2DSweep::updateFSM (include/private/solver2d.hpp:203-327).
+//
+// One workgroup owns one problem for the whole solve; levels are separated
+// by workgroup barriers.  2D grids are small (config of the reference's own
+// 2D tests: 201 x 201), so one launch covers the entire solve.
+#include "ekx_kernels.hpp"
+
+namespace ekx
+{
+
+// computeAnalyticalTravelTime 2D (solverUtilities2d.hpp:159-174).
+__device__ __forceinline__ double analytic_time2d(int ix, int iz, double dx,
+                                                  double dz, double xs,
+                                                  double zs, double slow)
+{
+    double x = ix*dx;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaZ = z - zs;
+    double distance = ::hypot(deltaX, deltaZ);
+    return slow*distance;
+}
+
+// (solverUtilities2d.hpp:192-215)
+__device__ __forceinline__ void analytic_time_deriv2d(
+    int ix, int iz, double dx, double dz, double xs, double zs, double slow,
+    double &t, double &dtdx, double &dtdz)
+{
+    double x = ix*dx;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaZ = z - zs;
+    double distance = ::hypot(deltaX, deltaZ);
+    t = slow*distance;
+    dtdx = 0;
+    dtdz = 0;
+    if (distance > 0)
+    {
+        dtdx = (slow*deltaX)/distance;
+        dtdz = (slow*deltaZ)/distance;
+    }
+}
+
+constexpr double kSqrt1_2 = 0.70710678118654752440;
+
+// finiteDifference, uniform grid (solverUtilities2d.hpp:345-472).
+__device__ double fd2d_uniform(const Problem2D &p, int ix, int iz, int signX,
+                               int signZ, int ixShift, int izShift,
+                               double s0, double s1, double s3, double t1,
+                               double t2, double t3)
+{
+    const double huge = kHuge;
+    const double h = p.dx;
+    const double sourceSlowness = p.srcSlowness;
+    double tUpd = huge;
+    double t0x = t1 + h*fmin(s0, s3);
+    double t0z = t3 + h*fmin(s0, s1);
+    double t0Update1 = fmin(t0x, t0z);
+    double hs0 = h*s0;
+    double dTx = t3 - t2;
+    double dTz = t1 - t2;
+    double sx = kSqrt1_2*s0;
+    double sz = kSqrt1_2*s0;
+    if (abs(p.srcCellX - ix) > p.radius || abs(p.srcCellZ - iz) > p.radius)
+    {
+        double t0a = huge;
+        double deta = hs0*hs0 - dTz*dTz;
+        if (dTz >= 0 && deta >= 0 && dTz <= h*sz)
+        {
+            t0a = t1 + h*::sqrt(deta);
+        }
+        double t0b = huge;
+        double detb = hs0*hs0 - dTx*dTx;
+        if (dTx >= 0 && detb >= 0 && dTx <= h*sx)
+        {
+            t0b = t3 + h*::sqrt(detb);
+        }
+        double t0Update2 = fmin(t0a, t0b);
+        double dtCross = t1 - t3;
+        double detc = 2*(hs0*hs0) - dtCross*dtCross;
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && detc > 0 &&
+            t3 <= t1 + hs0 && t1 <= t3 + hs0)
+        {
+            t0Update3 = t2 + ::sqrt(detc);
+        }
+        tUpd = fmin(fmin(t0Update1, t0Update2), t0Update3);
+    }
+    else
+    {
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && t1 < t3 + hs0 && t3 < t1 + hs0)
+        {
+            double dt0dx, dt0dz, t0;
+            const double xs = p.srcOffX, zs = p.srcOffZ;
+            analytic_time_deriv2d(ix, iz, h, h, xs, zs, sourceSlowness, t0,
+                                  dt0dx, dt0dz);
+            double tau1 = t1 - analytic_time2d(ix + ixShift, iz, h, h, xs, zs,
+                                               sourceSlowness);
+            double tau2 = t2 - analytic_time2d(ix + ixShift, iz + izShift, h,
+                                               h, xs, zs, sourceSlowness);
+            double tau3 = t3 - analytic_time2d(ix, iz + izShift, h, h, xs, zs,
+                                               sourceSlowness);
+            dt0dx = signX*dt0dx;
+            dt0dz = signZ*dt0dz;
+            double tauX = tau1 - tau3 + tau2;
+            double tauZ = tau3 - tau1 + tau2;
+            const double one = 1, two = 2, four = 4;
+            double h2 = h*h;
+            double a = one/h2;
+            double b = two/h*(dt0dx + dt0dz) - two/h2*tau2;
+            double c = one/(two*h2)*(tauX*tauX + tauZ*tauZ)
+                     - two/h*(dt0dx*tauX + dt0dz*tauZ)
+                     + two*(sourceSlowness*sourceSlowness - s0*s0);
+            double det = b*b - four*a*c;
+            if (det >= 0)
+            {
+                double tau = (-b + ::sqrt(det))/(two*a);
+                t0Update3 = t0 + tau;
+            }
+            if (t0Update3 < t1 || t0Update3 < t3) { t0Update3 = huge; }
+        }
+        tUpd = fmin(t0Update1, t0Update3);
+    }
+    return tUpd;
+}
+
+// finiteDifference, non-uniform grid (solverUtilities2d.hpp:474-614).
+__device__ double fd2d_nonuniform(const Problem2D &p, int ix, int iz,
+                                  int signX, int signZ, int ixShift,
+                                  int izShift, double s0, double s1,
+                                  double s3, double t1, double t2, double t3)
+{
+    const double huge = kHuge;
+    const double dx = p.dx, dz = p.dz;
+    const double dx_dz = p.dx_dz, dz_dx = p.dz_dx;
+    const double cosTheta = p.cosTheta, sinTheta = p.sinTheta;
+    const double sourceSlowness = p.srcSlowness;
+    double tUpd = huge;
+    double t0x = t1 + dx*fmin(s0, s3);
+    double t0z = t3 + dz*fmin(s0, s1);
+    double t0Update1 = fmin(t0x, t0z);
+    double dTx = t3 - t2;
+    double dTz = t1 - t2;
+    double sx = cosTheta*s0;
+    double sz = sinTheta*s0;
+    double hxs0 = dx*s0;
+    double hzs0 = dz*s0;
+    if (abs(p.srcCellX - ix) > p.radius || abs(p.srcCellZ - iz) > p.radius)
+    {
+        double t0a = huge;
+        double deta = hzs0*hzs0 - dTz*dTz;
+        if (dTz >= 0 && deta >= 0 && dTz <= dz*sz)
+        {
+            t0a = t1 + dx_dz*::sqrt(deta);
+        }
+        double t0b = huge;
+        double detb = hxs0*hxs0 - dTx*dTx;
+        if (dTx >= 0 && detb >= 0 && dTx <= dx*sx)
+        {
+            t0b = t3 + dz_dx*::sqrt(detb);
+        }
+        double t0Update2 = fmin(t0a, t0b);
+        double dxdz_dx2_p_dz2 = cosTheta*sinTheta;
+        double tildeTx = t1 - t3 + t2;
+        double tildeTz = t3 - t1 + t2;
+        double dtCross = t1 - t3;
+        double detc = hxs0*hxs0 + hzs0*hzs0 - dtCross*dtCross;
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && detc > 0 &&
+            t1 < t3 + hzs0 && t3 < t1 + hxs0)
+        {
+            t0Update3 = tildeTx*dz_dx + tildeTz*dx_dz + 2*::sqrt(detc);
+            t0Update3 = dxdz_dx2_p_dz2*t0Update3;
+        }
+        if (t0Update3 < 0) { t0Update3 = huge; }
+        tUpd = fmin(fmin(t0Update1, t0Update2), t0Update3);
+    }
+    else
+    {
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && t1 < t3 + hzs0 && t3 < t1 + hxs0)
+        {
+            double dt0dx, dt0dz, t0;
+            const double xs = p.srcOffX, zs = p.srcOffZ;
+            analytic_time_deriv2d(ix, iz, dx, dz, xs, zs, sourceSlowness, t0,
+                                  dt0dx, dt0dz);
+            double tau1 = t1 - analytic_time2d(ix + ixShift, iz, dx, dz, xs,
+                                               zs, sourceSlowness);
+            double tau2 = t2 - analytic_time2d(ix + ixShift, iz + izShift, dx,
+                                               dz, xs, zs, sourceSlowness);
+            double tau3 = t3 - analytic_time2d(ix, iz + izShift, dx, dz, xs,
+                                               zs, sourceSlowness);
+            dt0dx = signX*dt0dx;
+            dt0dz = signZ*dt0dz;
+            double tauX = tau1 - tau3 + tau2;
+            double tauZ = tau3 - tau1 + tau2;
+            const double one = 1, two = 2, four = 4;
+            double hx2inv = one/(dx*dx);
+            double hz2inv = one/(dz*dz);
+            double dxinv = one/dx;
+            double dzinv = one/dz;
+            double a = hx2inv + hz2inv;
+            double b = four*(dxinv*dt0dx + dzinv*dt0dz)
+                     - two*(tauX*hx2inv + tauZ*hz2inv);
+            double c = (tauX*tauX)*hx2inv + (tauZ*tauZ)*hz2inv
+                     - four*((dt0dx*tauX)*dxinv + (dt0dz*tauZ)*dzinv)
+                     + four*(sourceSlowness*sourceSlowness - s0*s0);
+            double det = b*b - four*a*c;
+            if (det >= 0)
+            {
+                double tau = (-b + ::sqrt(det))/(two*a);
+                t0Update3 = t0 + tau;
+            }
+            if (t0Update3 < t1 || t0Update3 < t3) { t0Update3 = huge; }
+        }
+        tUpd = fmin(t0Update1, t0Update3);
+    }
+    return tUpd;
+}
+
+// One sweep of one problem, levels separated by block barriers.
+__device__ void sweep2d_block(const Problem2D &p, int sweep, int initialize,
+                              double *__restrict__ T,
+                              const double *__restrict__ S)
+{
+    const int nx = p.nx, nz = p.nz, ncx = nx - 1, ncz = nz - 1;
+    const int negX = sweep & 1, negZ = (sweep >> 1) & 1;
+    const int signX = negX ? -1 : 1, signZ = negZ ? -1 : 1;
+    const int ixShift = negX ? 1 : -1, izShift = negZ ? 1 : -1;
+    int jx0 = 1, jz0 = 1;
+    if (initialize)
+    {
+        // getLoopLimits (init) 2D, solverUtilities2d.hpp:1004-1053
+        jx0 = negX ? max(nx - 2 - p.srcCellX, 1) : max(p.srcCellX, 1);
+        jz0 = negZ ? max(nz - 2 - p.srcCellZ, 1) : max(p.srcCellZ, 1);
+    }
+    const int lastLevel = (nx - 1) + (nz - 1);
+    for (int level = jx0 + jz0; level <= lastLevel; ++level)
+    {
+        const int kLo = max(jz0, level - (nx - 1));
+        const int kHi = min(nz - 1, level - jx0);
+        for (int jz = kLo + (int) threadIdx.x; jz <= kHi; jz += blockDim.x)
+        {
+            const int jx = level - jz;
+            const int ix = negX ? nx - 1 - jx : jx;
+            const int iz = negZ ? nz - 1 - jz : jz;
+            const int ixU = negX ? ix + 1 : ix - 1;
+            const int izU = negZ ? iz + 1 : iz - 1;
+            // gridToSurroundingSlowness (solverUtilities2d.hpp:1279-1370)
+            // in the permuted frame: home (jx-1, jz-1), offsets clamped.
+            const int pcxH = jx - 1, pcxO = min(jx, ncx - 1);
+            const int pczH = jz - 1, pczO = min(jz, ncz - 1);
+            const int cxH = negX ? ncx - 1 - pcxH : pcxH;
+            const int cxO = negX ? ncx - 1 - pcxO : pcxO;
+            const int czH = negZ ? ncz - 1 - pczH : pczH;
+            const int czO = negZ ? ncz - 1 - pczO : pczO;
+            const double s0 = S[czH*ncx + cxH];
+            const double s1 = S[czH*ncx + cxO];
+            const double s3 = S[czO*ncx + cxH];
+            const int i0 = iz*nx + ix;
+            const double t0 = T[i0];
+            const double t1 = T[iz*nx + ixU];
+            const double t2 = T[izU*nx + ixU];
+            const double t3 = T[izU*nx + ix];
+            double tUpd;
+            if (p.uniform)
+            {
+                tUpd = fd2d_uniform(p, ix, iz, signX, signZ, ixShift, izShift,
+                                    s0, s1, s3, t1, t2, t3);
+            }
+            else
+            {
+                tUpd = fd2d_nonuniform(p, ix, iz, signX, signZ, ixShift,
+                                       izShift, s0, s1, s3, t1, t2, t3);
+            }
+            if (tUpd < t0) { T[i0] = tUpd; }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+solve2d_kernel(const Problem2D *__restrict__ problems, double *__restrict__ T,
+               const double *__restrict__ S)
+{
+    const Problem2D &p = problems[blockIdx.x];
+    double *Tp = T + p.tOffset;
+    const double *Sp = S + p.sOffset;
+    const int n = p.nx*p.nz;
+    // Solver2DImpl::initializeTravelTimes (src/solver2d.cpp:61-105)
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        int iz = i/p.nx, ix = i - iz*p.nx;
+        int ox = ix - p.srcCellX, oz = iz - p.srcCellZ;
+        double v = kHuge;
+        if (ox >= 0 && ox <= 1 && oz >= 0 && oz <= 1)
+        {
+            v = analytic_time2d(ix, iz, p.dx, p.dz, p.srcOffX, p.srcOffZ,
+                                p.srcSlowness);
+        }
+        Tp[i] = v;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 4; ++sweep)
+    {
+        sweep2d_block(p, sweep, 1, Tp, Sp);
+    }
+    for (int k = 0; k < p.nSweeps; ++k)
+    {
+        for (int sweep = 0; sweep < 4; ++sweep)
+        {
+            sweep2d_block(p, sweep, 0, Tp, Sp);
+        }
+    }
+}
+
+// Solver2DImpl::interpolateTravelTimes (src/solver2d.cpp:106-167) with
+// bilinear (src/ray/bilinear.hpp).  Station (x, z) offsets and cells are
+// computed on the host exactly as Station2D does.
+__global__ void __launch_bounds__(256)
+stations2d_kernel(const Problem2D *__restrict__ problem, int n,
+                  const double *__restrict__ xOff,
+                  const double *__restrict__ zOff,
+                  const int *__restrict__ cellX, const int *__restrict__ cellZ,
+                  const double *__restrict__ T, double *__restrict__ out)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    const Problem2D &p = *problem;
+    const double *Tp = T + p.tOffset;
+    const double dx = p.dx, dz = p.dz;
+    const double dxi = 1./dx;
+    const double dzi = 1./dz;
+    const double xi = xOff[i], zi = zOff[i];
+    const int cx = cellX[i], cz = cellZ[i];
+    if (cx == p.srcCellX && cz == p.srcCellZ)
+    {
+        out[i] = ::hypot(xi - p.srcOffX, zi - p.srcOffZ)*p.srcSlowness;
+        return;
+    }
+    const int i00 = cz*p.nx + cx;
+    const int i10 = i00 + 1;
+    const int i01 = i00 + p.nx;
+    const int i11 = i01 + 1;
+    const double x1 = cx*dx;
+    const double x2 = (cx + 1)*dx;
+    const double z1 = cz*dz;
+    const double z2 = (cz + 1)*dz;
+    const double f00 = Tp[i00], f01 = Tp[i01], f10 = Tp[i10], f11 = Tp[i11];
+    double x2mx = x2 - xi;
+    double xmx1 = xi - x1;
+    double fxz1 = dxi*(x2mx*f00 + xmx1*f10);
+    double fxz2 = dxi*(x2mx*f01 + xmx1*f11);
+    double z2mz = z2 - zi;
+    double zmz1 = zi - z1;
+    out[i] = dzi*(z2mz*fxz1 + zmz1*fxz2);
+}
+
+__global__ void __launch_bounds__(256)
+nodal_to_cell2d(int nx, int nz, const double *__restrict__ v, int ordering,
+                double *__restrict__ slow)
+{
+    const int ncx = nx - 1, ncz = nz - 1;
+    for (int i = blockIdx.x*blockDim.x + threadIdx.x; i < ncx*ncz;
+         i += gridDim.x*blockDim.x)
+    {
+        int iz = i/ncx, ix = i - iz*ncx;
+        int idx1, idx2, idx3, idx4;
+        if (ordering == 0)
+        {
+            idx1 = iz*(ncx + 1) + ix;
+            idx2 = idx1 + 1;
+            idx3 = idx1 + (ncx + 1);
+            idx4 = idx3 + 1;
+        }
+        else
+        {
+            idx1 = ix*(ncz + 1) + iz;
+            idx2 = idx1 + 1;
+            idx3 = idx1 + (ncz + 1);
+            idx4 = idx3 + 1;
+        }
+        double sum1 = v[idx1] + v[idx2];
+        double sum2 = v[idx3] + v[idx4];
+        slow[i] = 4.0/(sum1 + sum2);
+    }
+}
+
+hipError_t launch_solve2d(const Problem2D *dProblems, int nProblems,
+                          double *T, const double *S, hipStream_t stream)
+{
+    hipLaunchKernelGGL(solve2d_kernel, dim3((unsigned) nProblems), dim3(256),
+                       0, stream, dProblems, T, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_stations2d(const Problem2D *dProblem, int n,
+                             const double *xOff, const double *zOff,
+                             const int *cellX, const int *cellZ,
+                             const double *T, double *out, hipStream_t stream)
+{
+    if (n <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(stations2d_kernel, dim3((unsigned) ((n + 255)/256)),
+                       dim3(256), 0, stream, dProblem, n, xOff, zOff, cellX,
+                       cellZ, T, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_nodal_to_cell2d(int nx, int nz, const double *v,
+                                  int ordering, double *slow,
+                                  hipStream_t stream)
+{
+    int nCell = (nx - 1)*(nz - 1);
+    unsigned g = (unsigned) ((nCell + 255)/256);
+    if (g > 4096) { g = 4096; }
+    hipLaunchKernelGGL(nodal_to_cell2d, dim3(g), dim3(256), 0, stream, nx, nz,
+                       v, ordering, slow);
+    return hipGetLastError();
+}
+
+} // namespace ekx

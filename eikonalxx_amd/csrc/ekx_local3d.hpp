@@ -1,0 +1,519 @@
+// ekx_local3d.hpp -- the 3D local (per-node) update on the device.
+//
+// This is the arithmetic of finiteDifference in
+// include/private/solverUtilities3d.hpp (uniform :258-558, non-uniform
+// :562-824) and computeAnalyticalTravelTime (:138-208).  Every expression
+// keeps the reference's operation order: the library is compiled with
+// -ffp-contract=off, so in double precision the result is the same value
+// the reference produces, bit for bit, given correctly rounded sqrt and
+// division.
+//
+// Neighbour naming follows the reference (in the sweep's own frame, where
+// the wave travels towards +x, +y, +z):
+//   t1 = T(x-1,y,z)  t2 = T(x-1,y-1,z)  t3 = T(x,y-1,z)  t4 = T(x,y,z-1)
+//   t5 = T(x-1,y,z-1) t6 = T(x-1,y-1,z-1) t7 = T(x,y-1,z-1)
+//   s0 = cell(x-1,y-1,z-1), s1 = (x,y-1,z-1), s2 = (x,y,z-1),
+//   s3 = (x-1,y,z-1), s4 = (x-1,y-1,z), s5 = (x,y-1,z), s7 = (x-1,y,z)
+// (cells indexed by their lower corner, clamped to the model).
+#pragma once
+#include "ekx_device.hpp"
+
+namespace ekx
+{
+
+template <typename R> struct Math;
+template <> struct Math<double>
+{
+    static __device__ __forceinline__ double min(double a, double b)
+    { return fmin(a, b); }
+    static __device__ __forceinline__ double max(double a, double b)
+    { return fmax(a, b); }
+    static __device__ __forceinline__ double sqrt(double a)
+    { return ::sqrt(a); }
+};
+template <> struct Math<float>
+{
+    static __device__ __forceinline__ float min(float a, float b)
+    { return fminf(a, b); }
+    static __device__ __forceinline__ float max(float a, float b)
+    { return fmaxf(a, b); }
+    // One-ulp hardware square root: the FP32 modes are validated against
+    // the FP64 solve at 1e-5 relative, far above this rounding.
+    static __device__ __forceinline__ float sqrt(float a)
+    { return __builtin_amdgcn_sqrtf(a); }
+};
+
+struct Sweep3D
+{
+    int signX, signY, signZ;        // getSweepFiniteDifferenceSigns
+    int ixShift, iyShift, izShift;
+};
+
+// Sweep k (0..7): bit0 x decreasing, bit1 y decreasing, bit2 z decreasing
+// (enums.hpp:66-76, solverUtilities3d.hpp:843-925).
+__device__ __forceinline__ Sweep3D make_sweep3d(int sweep)
+{
+    Sweep3D s;
+    int negX = sweep & 1, negY = (sweep >> 1) & 1, negZ = (sweep >> 2) & 1;
+    s.signX = negX ? -1 : 1;
+    s.signY = negY ? -1 : 1;
+    s.signZ = negZ ? -1 : 1;
+    s.ixShift = negX ? 1 : -1;
+    s.iyShift = negY ? 1 : -1;
+    s.izShift = negZ ? 1 : -1;
+    return s;
+}
+
+// computeAnalyticalTravelTime (solverUtilities3d.hpp:138-156).
+__device__ __forceinline__ double analytic_time3d(
+    int ix, int iy, int iz, double dx, double dy, double dz,
+    double xs, double ys, double zs, double slow)
+{
+    double x = ix*dx;
+    double y = iy*dy;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaY = y - ys;
+    double deltaZ = z - zs;
+    double distance = ::sqrt(deltaX*deltaX + deltaY*deltaY + deltaZ*deltaZ);
+    return slow*distance;
+}
+
+// computeAnalyticalTravelTime with derivatives (:180-208).
+__device__ __forceinline__ void analytic_time_deriv3d(
+    int ix, int iy, int iz, double dx, double dy, double dz,
+    double xs, double ys, double zs, double slow,
+    double &t, double &dtdx, double &dtdy, double &dtdz)
+{
+    double x = ix*dx;
+    double y = iy*dy;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaY = y - ys;
+    double deltaZ = z - zs;
+    double distance = ::sqrt(deltaX*deltaX + deltaY*deltaY + deltaZ*deltaZ);
+    t = slow*distance;
+    dtdx = 0;
+    dtdy = 0;
+    dtdz = 0;
+    if (distance > 0)
+    {
+        dtdx = (slow*deltaX)/distance;
+        dtdy = (slow*deltaY)/distance;
+        dtdz = (slow*deltaZ)/distance;
+    }
+}
+
+__device__ __forceinline__ bool in_factored_box(const Problem3D &p, int ix,
+                                                int iy, int iz)
+{
+    return !(abs(p.srcCellX - ix) > p.radius ||
+             abs(p.srcCellY - iy) > p.radius ||
+             abs(p.srcCellZ - iz) > p.radius);
+}
+
+// Cartesian branch, uniform grid (:281-371) in arithmetic R.
+template <typename R>
+__device__ __forceinline__ R cartesian_uniform3d(
+    R h, R s0, R s1, R s2, R s3, R s4, R s5, R s7,
+    R t1, R t2, R t3, R t4, R t5, R t6, R t7)
+{
+    using M = Math<R>;
+    const R huge = (R) kHuge;
+    R minS0S3 = M::min(s0, s3);
+    R minS0S1 = M::min(s0, s1);
+    R minS0S4 = M::min(s0, s4);
+    R t1d1 = t1 + h*M::min(minS0S3, M::min(s4, s7));
+    R t1d2 = t3 + h*M::min(minS0S4, M::min(s1, s5));
+    R t1d3 = t4 + h*M::min(minS0S1, M::min(s2, s3));
+    R t1d = M::min(M::min(t1d1, t1d2), t1d3);
+    R hs03 = h*minS0S3;
+    R hs01 = h*minS0S1;
+    R hs04 = h*minS0S4;
+    R dtCrossXZ = t1 - t4;
+    R detXZ = 2*(hs03*hs03) - dtCrossXZ*dtCrossXZ;
+    R t2d1 = huge;
+    if (detXZ > 0 && t4 <= t1 + hs03 && t1 <= t4 + hs03)
+    {
+        t2d1 = t5 + M::sqrt(detXZ);
+    }
+    R t2d = t2d1;
+    R dtCrossYZ = t3 - t4;
+    R detYZ = 2*(hs01*hs01) - dtCrossYZ*dtCrossYZ;
+    R t2d2 = huge;
+    if (detYZ > 0 && t3 <= t4 + hs01 && t4 <= t3 + hs01)
+    {
+        t2d2 = t7 + M::sqrt(detYZ);
+    }
+    t2d = M::min(t2d, t2d2);
+    R dtCrossXY = t1 - t3;
+    R detXY = 2*(hs04*hs04) - dtCrossXY*dtCrossXY;
+    R t2d3 = huge;
+    if (detXY > 0 && t3 <= t1 + hs04 && t1 <= t3 + hs04)
+    {
+        t2d3 = t2 + M::sqrt(detXY);
+    }
+    t2d = M::min(t2d, t2d3);
+    R t3d = huge;
+    if (M::min(t1d, t2d) > M::max(M::max(t1, t4), t3))
+    {
+        const R third = (R) 0.33333333333333333333333;
+        const R half = (R) 0.5;
+        R Tx = t1 + half*(-t4 + t5 - t3 + t2) - t7 + t6;
+        R Ty = t3 + half*(-t4 + t7 - t1 + t2) - t5 + t6;
+        R Tz = t4 + half*(-t1 + t5 - t3 + t7) - t2 + t6;
+        R det = 27*(h*s0)*(h*s0)
+              - (Tx - Ty)*(Tx - Ty)
+              - (Tx - Tz)*(Tx - Tz)
+              - (Ty - Tz)*(Ty - Tz);
+        if (det >= 0)
+        {
+            t3d = third*((Tx + Ty + Tz) + M::sqrt(det));
+        }
+    }
+    return M::min(t1d, M::min(t2d, t3d));
+}
+
+// Factored (spherical) branch, uniform grid (:281-294, :372-553); always
+// evaluated in double.
+__device__ __noinline__ double factored_uniform3d(
+    const Problem3D &p, const Sweep3D &sw, int ix, int iy, int iz,
+    double s0, double s1, double s2, double s3, double s4, double s5,
+    double s7, double t1, double t2, double t3, double t4, double t5,
+    double t6, double t7)
+{
+    const double huge = kHuge;
+    const double h = p.dx;
+    const double sourceSlowness = p.srcSlowness;
+    double minS0S3 = fmin(s0, s3);
+    double minS0S1 = fmin(s0, s1);
+    double minS0S4 = fmin(s0, s4);
+    double t1d1 = t1 + h*fmin(minS0S3, fmin(s4, s7));
+    double t1d2 = t3 + h*fmin(minS0S4, fmin(s1, s5));
+    double t1d3 = t4 + h*fmin(minS0S1, fmin(s2, s3));
+    double t1d = fmin(fmin(t1d1, t1d2), t1d3);
+    double hs03 = h*minS0S3;
+    double hs01 = h*minS0S1;
+    double hs04 = h*minS0S4;
+    const double xs = p.srcOffX, ys = p.srcOffY, zs = p.srcOffZ;
+    double t0, dt0dx, dt0dy, dt0dz;
+    analytic_time_deriv3d(ix, iy, iz, h, h, h, xs, ys, zs, s0,
+                          t0, dt0dx, dt0dy, dt0dz);
+    double tau1 = t1 - analytic_time3d(ix + sw.ixShift, iy, iz,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau2 = t2 - analytic_time3d(ix + sw.ixShift, iy + sw.iyShift, iz,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau3 = t3 - analytic_time3d(ix, iy + sw.iyShift, iz,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau4 = t4 - analytic_time3d(ix, iy, iz + sw.izShift,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau5 = t5 - analytic_time3d(ix + sw.ixShift, iy, iz + sw.izShift,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau6 = t6 - analytic_time3d(ix + sw.ixShift, iy + sw.iyShift,
+                                       iz + sw.izShift,
+                                       h, h, h, xs, ys, zs, s0);
+    double tau7 = t7 - analytic_time3d(ix, iy + sw.iyShift, iz + sw.izShift,
+                                       h, h, h, xs, ys, zs, s0);
+    dt0dx = sw.signX*dt0dx;
+    dt0dy = sw.signY*dt0dy;
+    dt0dz = sw.signZ*dt0dz;
+    double dTauTxXZ = tau1 - tau4 + tau5;
+    double dTauTzXZ = tau4 - tau1 + tau5;
+    const double half = 0.5;
+    double hi = 1/h;
+    double h2inv = 1/(h*h);
+    double a = h2inv;
+    double b = (2*hi)*(dt0dx + dt0dz)
+             - (2*h2inv)*tau5;
+    double c = (half*h2inv)*(dTauTxXZ*dTauTxXZ + dTauTzXZ*dTauTzXZ)
+             - (2*hi)*(dt0dx*dTauTxXZ + dt0dz*dTauTzXZ)
+             + 2*(sourceSlowness*sourceSlowness - s0*s0 + dt0dy*dt0dy);
+    double detXZ = b*b - 4*a*c;
+    double tau = 0;
+    double t2d1 = huge;
+    if (detXZ > 0 && t4 <= t1 + hs03 && t1 <= t4 + hs03)
+    {
+        tau = (-b + ::sqrt(detXZ))/(2*a);
+        t2d1 = t0 + tau;
+        if (t2d1 < t1 || t2d1 < t4) { t2d1 = huge; }
+    }
+    double t2d = t2d1;
+    double dTauTyYZ = tau3 - tau4 + tau7;
+    double dTauTzYZ = tau4 - tau3 + tau7;
+    b = (2*hi)*(dt0dy + dt0dz)
+      - (2*h2inv)*tau7;
+    c = (half*h2inv)*(dTauTyYZ*dTauTyYZ + dTauTzYZ*dTauTzYZ)
+      - (2*hi)*(dt0dy*dTauTyYZ + dt0dz*dTauTzYZ)
+      + 2*(sourceSlowness*sourceSlowness - s0*s0 + dt0dx*dt0dx);
+    double detYZ = b*b - 4*a*c;
+    double t2d2 = huge;
+    if (detYZ >= 0 && t3 <= t4 + hs01 && t4 <= t3 + hs01)
+    {
+        tau = (-b + ::sqrt(detYZ))/(2*a);
+        t2d2 = t0 + tau;
+        if (t2d2 < t3 || t2d2 < t4) { t2d2 = huge; }
+    }
+    t2d = fmin(t2d, t2d2);
+    double dTauTxXY = tau1 - tau3 + tau2;
+    double dTauTyXY = tau3 - tau1 + tau2;
+    b = (2*hi)*(dt0dx + dt0dy)
+      - (2*h2inv)*tau2;
+    c = (half*h2inv)*(dTauTxXY*dTauTxXY + dTauTyXY*dTauTyXY)
+      - (2*hi)*(dt0dx*dTauTxXY + dt0dy*dTauTyXY)
+      + 2*(sourceSlowness*sourceSlowness - s0*s0 + dt0dz*dt0dz);
+    double detXY = b*b - 4*a*c;
+    double t2d3 = huge;
+    if (detXY >= 0 && t3 <= t1 + hs04 && t1 <= t3 + hs04)
+    {
+        tau = (-b + ::sqrt(detXY))/(2*a);
+        t2d3 = t0 + tau;
+        if (t2d3 < t1 || t2d3 < t3) { t2d3 = huge; }
+    }
+    t2d = fmin(t2d, t2d3);
+    double t3d = huge;
+    if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+    {
+        double taux = tau1 + half*(-tau4 + tau5 - tau3 + tau2) - tau7 + tau6;
+        double tauy = tau3 + half*(-tau4 + tau7 - tau1 + tau2) - tau5 + tau6;
+        double tauz = tau4 + half*(-tau1 + tau5 - tau3 + tau7) - tau2 + tau6;
+        a = 3*h2inv;
+        b =-2*h2inv*(taux + tauy + tauz)
+          + (6*hi)*(dt0dx + dt0dy + dt0dz);
+        c = h2inv*(taux*taux + tauy*tauy + tauz*tauz)
+          - (6*hi)*(dt0dx*taux + dt0dy*tauy + dt0dz*tauz)
+          + 9*(sourceSlowness*sourceSlowness - s0*s0);
+        double det = b*b - 4*a*c;
+        if (det >= 0)
+        {
+            t3d = t0 + (-b + ::sqrt(det))/(2*a);
+        }
+        if (t3d < t1 || t3d < t3 || t3d < t4) { t3d = huge; }
+    }
+    return fmin(t1d, fmin(t2d, t3d));
+}
+
+// Cartesian branch, non-uniform grid (:591-680) in arithmetic R.
+template <typename R>
+__device__ __forceinline__ R cartesian_nonuniform3d(
+    const Problem3D &p, R s0, R s1, R s2, R s3, R s4, R s5, R s7,
+    R t1, R t2, R t3, R t4, R t5, R t6, R t7)
+{
+    using M = Math<R>;
+    const R huge = (R) kHuge;
+    const R dx = (R) p.dx, dy = (R) p.dy, dz = (R) p.dz;
+    const NonUniform3D &nu = p.nu;
+    const R dx2Inv = (R) nu.dx2Inv, dy2Inv = (R) nu.dy2Inv;
+    const R dz2Inv = (R) nu.dz2Inv;
+    const R dx_dz = (R) nu.dx_dz, dz_dx = (R) nu.dz_dx;
+    const R dy_dz = (R) nu.dy_dz, dz_dy = (R) nu.dz_dy;
+    const R dx_dy = (R) nu.dx_dy, dy_dx = (R) nu.dy_dx;
+    const R dx2_p_dz2_inv = (R) nu.dx2_p_dz2_inv;
+    const R dy2_p_dz2_inv = (R) nu.dy2_p_dz2_inv;
+    const R dx2_p_dy2_inv = (R) nu.dx2_p_dy2_inv;
+    R minS0S3 = M::min(s0, s3);
+    R minS0S1 = M::min(s0, s1);
+    R minS0S4 = M::min(s0, s4);
+    R t1d1 = t1 + dx*M::min(minS0S3, M::min(s4, s7));
+    R t1d2 = t3 + dy*M::min(minS0S4, M::min(s1, s5));
+    R t1d3 = t4 + dz*M::min(minS0S1, M::min(s2, s3));
+    R t1d = M::min(M::min(t1d1, t1d2), t1d3);
+    R dtCrossXZ = t1 - t4;
+    R detXZ = (dx*dx + dz*dz)*(minS0S3*minS0S3) - dtCrossXZ*dtCrossXZ;
+    R t2d1 = huge;
+    if (detXZ > 0 && t1 < t4 + dz*minS0S3 && t4 < t1 + dx*minS0S3)
+    {
+        R TxXZ = t1 - t4 + t5;
+        R TzXZ = t4 - t1 + t5;
+        t2d1 = (TxXZ*dz_dx + TzXZ*dx_dz) + 2*M::sqrt(detXZ);
+        t2d1 = ((dx*dz)*dx2_p_dz2_inv)*t2d1;
+    }
+    R t2d = t2d1;
+    R dtCrossYZ = t3 - t4;
+    R detYZ = (dy*dy + dz*dz)*(minS0S1*minS0S1) - dtCrossYZ*dtCrossYZ;
+    R t2d2 = huge;
+    if (detYZ > 0 && t3 < t4 + dz*minS0S1 && t4 < t3 + dy*minS0S1)
+    {
+        R TyYZ = t3 - t4 + t7;
+        R TzYZ = t4 - t3 + t7;
+        t2d2 = (TyYZ*dz_dy + TzYZ*dy_dz) + 2*M::sqrt(detYZ);
+        t2d2 = ((dy*dz)*dy2_p_dz2_inv)*t2d2;
+    }
+    t2d = M::min(t2d, t2d2);
+    R dtCrossXY = t1 - t3;
+    R detXY = (dx*dx + dy*dy)*(minS0S4*minS0S4) - dtCrossXY*dtCrossXY;
+    R t2d3 = huge;
+    if (detXY > 0 && t3 < t1 + dx*minS0S4 && t1 < t3 + dy*minS0S4)
+    {
+        R TxXY = t1 - t3 + t2;
+        R TyXY = t3 - t1 + t2;
+        t2d3 = (TxXY*dy_dx + TyXY*dx_dy) + 2*M::sqrt(detXY);
+        t2d3 = ((dx*dy)*dx2_p_dy2_inv)*t2d3;
+    }
+    t2d = M::min(t2d, t2d3);
+    R t3d = huge;
+    if (M::min(t1d, t2d) > M::max(M::max(t1, t4), t3))
+    {
+        const R half = (R) 0.5;
+        R Tx = t1 + half*(-t4 + t5 - t3 + t2) - t7 + t6;
+        R Ty = t3 + half*(-t4 + t7 - t1 + t2) - t5 + t6;
+        R Tz = t4 + half*(-t1 + t5 - t3 + t7) - t2 + t6;
+        R det = 9*(s0*s0)*(dx2Inv + dy2Inv + dz2Inv)
+              - ((Tx - Tz)*(Tx - Tz))*(dx2Inv*dz2Inv)
+              - ((Ty - Tz)*(Ty - Tz))*(dy2Inv*dz2Inv)
+              - ((Tx - Ty)*(Tx - Ty))*(dx2Inv*dy2Inv);
+        if (det >= 0)
+        {
+            t3d = (Tx*dx2Inv + Ty*dy2Inv + Tz*dz2Inv + M::sqrt(det))
+                 /(dx2Inv + dy2Inv + dz2Inv);
+        }
+    }
+    return M::min(t1d, M::min(t2d, t3d));
+}
+
+// Factored branch, non-uniform grid (:591-599, :681-821), in double.
+__device__ __noinline__ double factored_nonuniform3d(
+    const Problem3D &p, const Sweep3D &sw, int ix, int iy, int iz,
+    double s0, double s1, double s2, double s3, double s4, double s5,
+    double s7, double t1, double t2, double t3, double t4, double t5,
+    double t6, double t7)
+{
+    const double huge = kHuge;
+    const double dx = p.dx, dy = p.dy, dz = p.dz;
+    const double sourceSlowness = p.srcSlowness;
+    const NonUniform3D &nu = p.nu;
+    const double dxInv = nu.dxInv, dyInv = nu.dyInv, dzInv = nu.dzInv;
+    const double dx2Inv = nu.dx2Inv, dy2Inv = nu.dy2Inv, dz2Inv = nu.dz2Inv;
+    double minS0S3 = fmin(s0, s3);
+    double minS0S1 = fmin(s0, s1);
+    double minS0S4 = fmin(s0, s4);
+    double t1d1 = t1 + dx*fmin(minS0S3, fmin(s4, s7));
+    double t1d2 = t3 + dy*fmin(minS0S4, fmin(s1, s5));
+    double t1d3 = t4 + dz*fmin(minS0S1, fmin(s2, s3));
+    double t1d = fmin(fmin(t1d1, t1d2), t1d3);
+    const double xs = p.srcOffX, ys = p.srcOffY, zs = p.srcOffZ;
+    double t0, dt0dx, dt0dy, dt0dz;
+    analytic_time_deriv3d(ix, iy, iz, dx, dy, dz, xs, ys, zs, s0,
+                          t0, dt0dx, dt0dy, dt0dz);
+    double tau1 = t1 - analytic_time3d(ix + sw.ixShift, iy, iz,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau2 = t2 - analytic_time3d(ix + sw.ixShift, iy + sw.iyShift, iz,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau3 = t3 - analytic_time3d(ix, iy + sw.iyShift, iz,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau4 = t4 - analytic_time3d(ix, iy, iz + sw.izShift,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau5 = t5 - analytic_time3d(ix + sw.ixShift, iy, iz + sw.izShift,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau6 = t6 - analytic_time3d(ix + sw.ixShift, iy + sw.iyShift,
+                                       iz + sw.izShift,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    double tau7 = t7 - analytic_time3d(ix, iy + sw.iyShift, iz + sw.izShift,
+                                       dx, dy, dz, xs, ys, zs, s0);
+    dt0dx = sw.signX*dt0dx;
+    dt0dy = sw.signY*dt0dy;
+    dt0dz = sw.signZ*dt0dz;
+    double dTauTxXZ = tau1 - tau4 + tau5;
+    double dTauTzXZ = tau4 - tau1 + tau5;
+    double a = dx2Inv + dz2Inv;
+    double b = 4*(dxInv*dt0dx + dzInv*dt0dz)
+             - 2*(dTauTxXZ*dx2Inv + dTauTzXZ*dz2Inv);
+    double c = (dTauTxXZ*dTauTxXZ)*dx2Inv + (dTauTzXZ*dTauTzXZ)*dz2Inv
+             - 4*((dt0dx*dTauTxXZ)*dxInv + (dt0dz*dTauTzXZ)*dzInv)
+             + 4*(sourceSlowness*sourceSlowness - s0*s0 + dt0dy*dt0dy);
+    double detXZ = b*b - 4*a*c;
+    double t2d1 = huge;
+    if (detXZ > 0 && t1 < t4 + dz*minS0S3 && t4 < t1 + dx*minS0S3)
+    {
+        t2d1 = t0 + (-b + ::sqrt(detXZ))/(2*a);
+        if (t2d1 < t1 || t2d1 < t4) { t2d1 = huge; }
+    }
+    double t2d = t2d1;
+    double dTauTyYZ = tau3 - tau4 + tau7;
+    double dTauTzYZ = tau4 - tau3 + tau7;
+    a = dy2Inv + dz2Inv;
+    b = 4*(dyInv*dt0dy + dzInv*dt0dz)
+      - 2*(dTauTyYZ*dy2Inv + dTauTzYZ*dz2Inv);
+    c = (dTauTyYZ*dTauTyYZ)*dy2Inv + (dTauTzYZ*dTauTzYZ)*dz2Inv
+      - 4*((dt0dy*dTauTyYZ)*dyInv + (dt0dz*dTauTzYZ)*dzInv)
+      + 4*(sourceSlowness*sourceSlowness - s0*s0 + dt0dx*dt0dx);
+    double detYZ = b*b - 4*a*c;
+    double t2d2 = huge;
+    if (detYZ > 0 && t3 < t4 + dz*minS0S1 && t4 < t3 + dy*minS0S1)
+    {
+        t2d2 = t0 + (-b + ::sqrt(detYZ))/(2*a);
+        if (t2d2 < t3 || t2d2 < t4) { t2d2 = huge; }
+    }
+    t2d = fmin(t2d, t2d2);
+    double dTauTxXY = tau1 - tau3 + tau2;
+    double dTauTyXY = tau3 - tau1 + tau2;
+    a = dx2Inv + dy2Inv;
+    b = 4*(dxInv*dt0dx + dyInv*dt0dy)
+      - 2*(dTauTxXY*dx2Inv + dTauTyXY*dy2Inv);
+    c = (dTauTxXY*dTauTxXY)*dx2Inv + (dTauTyXY*dTauTyXY)*dy2Inv
+      - 4*((dt0dx*dTauTxXY)*dxInv + (dt0dy*dTauTyXY)*dyInv)
+      + 4*(sourceSlowness*sourceSlowness - s0*s0 + dt0dz*dt0dz);
+    double detXY = b*b - 4*a*c;
+    double t2d3 = huge;
+    if (detXY > 0 && t3 < t1 + dx*minS0S4 && t1 < t3 + dy*minS0S4)
+    {
+        t2d3 = t0 + (-b + ::sqrt(detXY))/(2*a);
+        if (t2d3 < t1 || t2d3 < t3) { t2d3 = huge; }
+    }
+    t2d = fmin(t2d, t2d3);
+    double t3d = huge;
+    if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+    {
+        const double half = 0.5;
+        double taux = tau1 + half*(-tau4 + tau5 - tau3 + tau2) - tau7 + tau6;
+        double tauy = tau3 + half*(-tau4 + tau7 - tau1 + tau2) - tau5 + tau6;
+        double tauz = tau4 + half*(-tau1 + tau5 - tau3 + tau7) - tau2 + tau6;
+        a = dx2Inv + dy2Inv + dz2Inv;
+        b =-2*(taux*dx2Inv + tauy*dy2Inv + tauz*dz2Inv)
+          + 6*(dxInv*dt0dx + dyInv*dt0dy + dzInv*dt0dz);
+        c = (taux*taux)*dx2Inv + (tauy*tauy)*dy2Inv + (tauz*tauz)*dz2Inv
+          - 6*(dt0dx*taux*dxInv + dt0dy*tauy*dyInv + dt0dz*tauz*dzInv)
+          + 9*(sourceSlowness*sourceSlowness - s0*s0);
+        double det = b*b - 4*a*c;
+        if (det >= 0)
+        {
+            t3d = t0 + (-b + ::sqrt(det))/(2*a);
+        }
+        if (t3d < t1 || t3d < t3 || t3d < t4) { t3d = huge; }
+    }
+    return fmin(t1d, fmin(t2d, t3d));
+}
+
+// The complete local update: the factored branch inside the
+// (2 radius + 1)^3 box around the source cell, Cartesian outside
+// (:296-298).  ix, iy, iz are natural grid indices.
+template <typename R>
+__device__ __forceinline__ R local_update3d(
+    const Problem3D &p, const Sweep3D &sw, int ix, int iy, int iz,
+    R s0, R s1, R s2, R s3, R s4, R s5, R s7,
+    R t1, R t2, R t3, R t4, R t5, R t6, R t7)
+{
+    if (!in_factored_box(p, ix, iy, iz))
+    {
+        if (p.uniform)
+        {
+            return cartesian_uniform3d<R>((R) p.dx, s0, s1, s2, s3, s4, s5,
+                                          s7, t1, t2, t3, t4, t5, t6, t7);
+        }
+        return cartesian_nonuniform3d<R>(p, s0, s1, s2, s3, s4, s5, s7,
+                                         t1, t2, t3, t4, t5, t6, t7);
+    }
+    double r;
+    if (p.uniform)
+    {
+        r = factored_uniform3d(p, sw, ix, iy, iz, s0, s1, s2, s3, s4, s5, s7,
+                               t1, t2, t3, t4, t5, t6, t7);
+    }
+    else
+    {
+        r = factored_nonuniform3d(p, sw, ix, iy, iz, s0, s1, s2, s3, s4, s5,
+                                  s7, t1, t2, t3, t4, t5, t6, t7);
+    }
+    return (R) r;
+}
+
+} // namespace ekx

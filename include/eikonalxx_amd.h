@@ -1,0 +1,294 @@
+/*
+ * eikonalxx_amd.h -- C ABI of the MI355X (gfx950) fast-sweeping eikonal
+ * solver.  Plain pointers and sizes only; no C++ or device types cross
+ * this boundary.  All travel-time fields are nodal, x fastest, then y,
+ * then z (the reference's include/private/grid.hpp:27-34); slowness
+ * fields are cell based with the same ordering over cells
+ * (src/model3d.cpp:90).
+ *
+ * Every entry point returns an ekx_status; ekx_last_error() returns the
+ * message of the last failure on the calling thread.  No exception crosses
+ * the ABI: EKX_EINVAL corresponds to the reference's std::invalid_argument
+ * and EKX_ESTATE to its std::runtime_error (call-order violations).
+ *
+ * Handles are not thread safe; use one handle per host thread.  Each handle
+ * owns one HIP stream and its device buffers.
+ *
+ * Citations are to files of uofuseismo/eikonalxx.
+ */
+#ifndef EIKONALXX_AMD_H
+#define EIKONALXX_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EKX_API __attribute__((visibility("default")))
+
+typedef enum
+{
+    EKX_OK = 0,
+    EKX_EINVAL = 1,   /* invalid argument (std::invalid_argument) */
+    EKX_ESTATE = 2,   /* wrong call order (std::runtime_error)     */
+    EKX_EHIP = 3,     /* HIP runtime failure                        */
+    EKX_ENOMEM = 4    /* device or host allocation failed           */
+} ekx_status;
+
+/* include/eikonalxx/enums.hpp:44 */
+typedef enum
+{
+    EKX_LEVEL_SET_METHOD = 0,
+    EKX_FAST_SWEEPING_METHOD = 1
+} ekx_algorithm;
+
+/* include/eikonalxx/enums.hpp:23 (Ordering3D) / :7 (Ordering2D) */
+typedef enum
+{
+    EKX_ORDERING_NATURAL = 0, /* x fastest (ZYX / ZX) */
+    EKX_ORDERING_XYZ = 1      /* z fastest (XYZ / XZ) */
+} ekx_ordering;
+
+/* Arithmetic of a solve (no reference counterpart: the reference only
+ * instantiates Solver3D<double>, src/solver3d.cpp:626). */
+typedef enum
+{
+    EKX_PRECISION_FP64 = 0,        /* every sweep in double                */
+    EKX_PRECISION_FP32_POLISH = 1, /* FP32 sweeps, FP64 near the source,
+                                      final iteration (8 sweeps) in FP64   */
+    EKX_PRECISION_FP32 = 2         /* FP32 sweeps, FP64 near the source    */
+} ekx_precision;
+
+/* Sweep schedule of Solver3D::solve.  COMPLETED: 8 initialisation sweeps
+ * then number_of_sweeps x 8 refinement sweeps (the schedule of
+ * Solver2D::solve, src/solver2d.cpp:514-539, and of the commented-out lines
+ * src/solver3d.cpp:465-485).  SHIPPED: what src/solver3d.cpp:451-499 runs
+ * today (fast sweeping: initialisation sweeps 1 and 2 only; level set:
+ * source initialisation only). */
+typedef enum
+{
+    EKX_SCHEDULE_COMPLETED = 0,
+    EKX_SCHEDULE_SHIPPED = 1
+} ekx_schedule;
+
+/* Device kernel family for the 3D sweeps.  Both are exact topological
+ * orderings of the serial fast-sweeping loop and give identical results. */
+typedef enum
+{
+    EKX_KERNEL_AUTO = 0,
+    EKX_KERNEL_HYPERPLANE = 1, /* one launch per i+j+k hyperplane        */
+    EKX_KERNEL_COLUMN = 2      /* 8x8 columns, skewed z pipeline in LDS,
+                                  one launch per column diagonal         */
+} ekx_kernel;
+
+/* SolverOptions (include/eikonalxx/solverOptions.hpp:54-97, defaults
+ * src/solverOptions.cpp:5-21) plus the device-side choices. */
+typedef struct
+{
+    int number_of_sweeps;          /* refinement iterations, default 5     */
+    int factored_radius;           /* grid points, default 3, <0 disables  */
+    int algorithm;                 /* ekx_algorithm, default level set     */
+    int verbosity;                 /* 0 error .. 3 debug                   */
+    double convergence_tolerance;  /* seconds, default 1e-5                */
+    int precision;                 /* ekx_precision, default FP64          */
+    int schedule;                  /* ekx_schedule, default COMPLETED      */
+    int kernel;                    /* ekx_kernel, default AUTO             */
+    int early_exit;                /* 0 (default): always run every
+                                      iteration, as the reference does;
+                                      1: stop refinement iterations once the
+                                      largest change is below the
+                                      tolerance (FP32 modes only)          */
+    int use_graph;                 /* 1 (default): replay solves from a
+                                      captured hipGraph                     */
+} ekx_solver_options;
+
+/* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */
+typedef struct
+{
+    int nx, ny, nz;       /* grid points, each >= 3 */
+    double dx, dy, dz;    /* grid spacing (m), > 0  */
+    double x0, y0, z0;    /* origin (m)             */
+} ekx_geometry3d;
+
+/* Geometry2D (include/eikonalxx/geometry2d.hpp). */
+typedef struct
+{
+    int nx, nz;
+    double dx, dz;
+    double x0, z0;
+} ekx_geometry2d;
+
+/* Work and timing of the last solve on a handle. */
+typedef struct
+{
+    int64_t updates;          /* local-solver evaluations */
+    int64_t updates_fp64;     /* of which evaluated in FP64 sweeps        */
+    int64_t kernel_launches;  /* sweep kernel launches                    */
+    int32_t iterations_run;   /* refinement iterations actually run       */
+    int32_t reserved;
+    double last_solve_ms;     /* device time of the last solve (events)   */
+} ekx_solve_stats;
+
+/* ---------------------------------------------------------------------- */
+EKX_API const char *ekx_last_error(void);
+EKX_API const char *ekx_version(void);
+/* SolverOptions::SolverOptions defaults (src/solverOptions.cpp:5-21). */
+EKX_API void ekx_solver_options_default(ekx_solver_options *options);
+/* Number of visible HIP devices; 0 on a host without a GPU. */
+EKX_API ekx_status ekx_device_count(int *count);
+/* Blocks until all work of this process on the current device is done. */
+EKX_API ekx_status ekx_device_synchronize(void);
+
+/* ---------------------------- Solver3D ---------------------------------
+ * Replaces EikonalXX::Solver3D<double> (include/eikonalxx/solver3d.hpp:
+ * 19-177, src/solver3d.cpp). */
+typedef struct ekx_solver3d ekx_solver3d;
+
+/* Solver3D::Solver3D (src/solver3d.cpp:109-113). */
+EKX_API ekx_status ekx_solver3d_create(ekx_solver3d **solver);
+/* Solver3D::~Solver3D (src/solver3d.cpp:116-117). */
+EKX_API void ekx_solver3d_destroy(ekx_solver3d *solver);
+/* Solver3D::initialize(SolverOptions, Geometry3D)
+ * (solver3d.hpp:40-41, src/solver3d.cpp:148-194). */
+EKX_API ekx_status ekx_solver3d_initialize(ekx_solver3d *solver,
+                                           const ekx_solver_options *options,
+                                           const ekx_geometry3d *geometry);
+/* Solver3D::isInitialized (src/solver3d.cpp:197-201). */
+EKX_API int ekx_solver3d_is_initialized(const ekx_solver3d *solver);
+/* Solver3D::setVelocityModel(Model3D) with the model's cell slowness
+ * (Model3D::getSlownessPointer, src/model3d.cpp:401-409;
+ * Solver3D::setVelocityModel src/solver3d.cpp:212-249).  n_cells must be
+ * (nx-1)(ny-1)(nz-1); every slowness must be positive. */
+EKX_API ekx_status ekx_solver3d_set_slowness(ekx_solver3d *solver,
+                                             int64_t n_cells,
+                                             const double *slowness);
+/* Model3D::setNodalVelocities then Solver3D::setVelocityModel
+ * (src/model3d.cpp:285-314, interpolate3d :45-151); the nodal to cell
+ * conversion runs on the device. */
+EKX_API ekx_status ekx_solver3d_set_nodal_velocities(ekx_solver3d *solver,
+                                                     int64_t n_grid,
+                                                     const double *velocity,
+                                                     int ordering);
+/* Model3D::setCellularVelocities then Solver3D::setVelocityModel
+ * (src/model3d.cpp:316-375). */
+EKX_API ekx_status ekx_solver3d_set_cell_velocities(ekx_solver3d *solver,
+                                                    int64_t n_cells,
+                                                    const double *velocity,
+                                                    int ordering);
+/* Solver3D::haveVelocityModel (src/solver3d.cpp:275-279). */
+EKX_API int ekx_solver3d_have_velocity_model(const ekx_solver3d *solver);
+/* Solver3D::setSource(std::tuple<double,double,double>)
+ * (solver3d.hpp:97, src/solver3d.cpp:282-341 with Source3D
+ * src/source3d.cpp:161-291). */
+EKX_API ekx_status ekx_solver3d_set_source(ekx_solver3d *solver, double x,
+                                           double y, double z);
+/* Solver3D::haveSource (src/solver3d.cpp:385-389). */
+EKX_API int ekx_solver3d_have_source(const ekx_solver3d *solver);
+/* Source3D offsets and cell of the current source (src/source3d.cpp:
+ * 192-208, 247-263, 309-325, 143-158). */
+EKX_API ekx_status ekx_solver3d_get_source_cell(const ekx_solver3d *solver,
+                                                double offset[3],
+                                                int cell[3], int *cell_index);
+/* Solver3D::solve (solver3d.hpp:127, src/solver3d.cpp:391-500). */
+EKX_API ekx_status ekx_solver3d_solve(ekx_solver3d *solver);
+/* Solver3D::haveTravelTimeField (src/solver3d.cpp:510-514). */
+EKX_API int ekx_solver3d_have_travel_time_field(const ekx_solver3d *solver);
+/* Solver3D::getTravelTimeField (solver3d.hpp:137, src/solver3d.cpp:
+ * 503-507): copies the nx*ny*nz field (natural order) into out. */
+EKX_API ekx_status ekx_solver3d_get_travel_times(const ekx_solver3d *solver,
+                                                 int64_t n, double *out);
+/* Same field as float (the stored precision of the FP32 modes). */
+EKX_API ekx_status ekx_solver3d_get_travel_times_f32(
+    const ekx_solver3d *solver, int64_t n, float *out);
+/* Device pointer of the travel-time field (borrowed until the next solve or
+ * initialize), and its element size (4 or 8).  Corresponds to
+ * Solver3D::getTravelTimeFieldPointer (src/solver3d.cpp:516-525). */
+EKX_API ekx_status ekx_solver3d_get_travel_times_device(
+    const ekx_solver3d *solver, const void **device_ptr,
+    int *element_size);
+EKX_API ekx_status ekx_solver3d_get_stats(const ekx_solver3d *solver,
+                                          ekx_solve_stats *stats);
+/* The handle's HIP stream (hipStream_t), for callers that time or order
+ * work around a solve. */
+EKX_API ekx_status ekx_solver3d_get_stream(const ekx_solver3d *solver,
+                                           void **stream);
+
+/* ------------------------------ Batch3D --------------------------------
+ * Many independent 3D solves in one schedule of launches (no reference
+ * counterpart: the reference solves one source per Solver3D).  A problem
+ * is (geometry, slowness, source).  Problems whose slowness pointers and
+ * geometries are identical share one device copy of the model. */
+typedef struct
+{
+    ekx_geometry3d geometry;
+    const double *slowness;   /* host, (nx-1)(ny-1)(nz-1) cells */
+    double source[3];         /* absolute (x, y, z) in m        */
+} ekx_problem3d;
+
+typedef struct ekx_batch3d ekx_batch3d;
+
+EKX_API ekx_status ekx_batch3d_create(ekx_batch3d **batch);
+EKX_API void ekx_batch3d_destroy(ekx_batch3d *batch);
+EKX_API ekx_status ekx_batch3d_initialize(ekx_batch3d *batch,
+                                          const ekx_solver_options *options,
+                                          int n_problems,
+                                          const ekx_problem3d *problems);
+/* Moves problem i's source (same checks as ekx_solver3d_set_source). */
+EKX_API ekx_status ekx_batch3d_set_source(ekx_batch3d *batch, int problem,
+                                          double x, double y, double z);
+EKX_API ekx_status ekx_batch3d_solve(ekx_batch3d *batch);
+EKX_API ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *batch,
+                                                int problem, int64_t n,
+                                                double *out);
+EKX_API ekx_status ekx_batch3d_get_stats(const ekx_batch3d *batch,
+                                         ekx_solve_stats *stats);
+EKX_API ekx_status ekx_batch3d_get_stream(const ekx_batch3d *batch,
+                                          void **stream);
+
+/* ---------------------------- Solver2D ---------------------------------
+ * Replaces EikonalXX::Solver2D<double> (include/eikonalxx/solver2d.hpp,
+ * src/solver2d.cpp); the fast-sweeping schedule of src/solver2d.cpp:
+ * 462-565. */
+typedef struct ekx_solver2d ekx_solver2d;
+
+EKX_API ekx_status ekx_solver2d_create(ekx_solver2d **solver);
+EKX_API void ekx_solver2d_destroy(ekx_solver2d *solver);
+/* Solver2D::initialize (src/solver2d.cpp:250-302). */
+EKX_API ekx_status ekx_solver2d_initialize(ekx_solver2d *solver,
+                                           const ekx_solver_options *options,
+                                           const ekx_geometry2d *geometry);
+/* Solver2D::setVelocityModel with Model2D's cell slowness
+ * (src/solver2d.cpp:413-441). */
+EKX_API ekx_status ekx_solver2d_set_slowness(ekx_solver2d *solver,
+                                             int64_t n_cells,
+                                             const double *slowness);
+/* Model2D::setNodalVelocities (src/model2d.cpp:46-127) then
+ * Solver2D::setVelocityModel. */
+EKX_API ekx_status ekx_solver2d_set_nodal_velocities(ekx_solver2d *solver,
+                                                     int64_t n_grid,
+                                                     const double *velocity,
+                                                     int ordering);
+/* Solver2D::setSource(std::pair<double,double>) (src/solver2d.cpp:
+ * 351-392). */
+EKX_API ekx_status ekx_solver2d_set_source(ekx_solver2d *solver, double x,
+                                           double z);
+/* Solver2D::setStations (receiver locations, absolute x and z). */
+EKX_API ekx_status ekx_solver2d_set_stations(ekx_solver2d *solver, int n,
+                                             const double *x,
+                                             const double *z);
+/* Solver2D::solve (src/solver2d.cpp:462-565), including the bilinear
+ * interpolation at the stations (:106-167). */
+EKX_API ekx_status ekx_solver2d_solve(ekx_solver2d *solver);
+EKX_API ekx_status ekx_solver2d_get_travel_times(const ekx_solver2d *solver,
+                                                 int64_t n, double *out);
+/* Solver2D::getTravelTimesAtStations. */
+EKX_API ekx_status ekx_solver2d_get_travel_times_at_stations(
+    const ekx_solver2d *solver, int n, double *out);
+EKX_API ekx_status ekx_solver2d_get_stats(const ekx_solver2d *solver,
+                                          ekx_solve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,398 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package; it is the checker, never the product.
+The oracle restates the reference arithmetic of uofuseismo/eikonalxx in C
+(oracle/ekx_oracle.c, built by oracle/Makefile with -ffp-contract=off).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libekx_oracle.so")
+_lib = None
+
+SCHEDULE_COMPLETED = 0
+SCHEDULE_SHIPPED = 1
+ALGORITHM_LEVEL_SET = 0
+ALGORITHM_FAST_SWEEPING = 1
+
+
+class Geometry3D(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x0", ctypes.c_double), ("y0", ctypes.c_double),
+                ("z0", ctypes.c_double)]
+
+
+class Geometry2D(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double),
+                ("x0", ctypes.c_double), ("z0", ctypes.c_double)]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+
+def _declare(L):
+    L.ekxo_fd3d_uniform.restype = _D
+    L.ekxo_fd3d_uniform.argtypes = [_I, _D, _D, _D, _I, _I, _I, _PI, _PI,
+                                    _PI, _PD, _PD, _PD]
+    L.ekxo_fd3d_nonuniform.restype = _D
+    L.ekxo_fd3d_nonuniform.argtypes = [_I, _D, _D, _D, _D, _D, _I, _I, _I,
+                                       _PI, _PI, _PI, _PD, _PD, _PD]
+    L.ekxo_fd2d_uniform.restype = _D
+    L.ekxo_fd2d_uniform.argtypes = [_I, _D, _D, _D, _I, _I, _PI, _PI, _PI,
+                                    _PD, _PD, _PD]
+    L.ekxo_fd2d_nonuniform.restype = _D
+    L.ekxo_fd2d_nonuniform.argtypes = [_I, _D, _D, _D, _D, _I, _I, _PI, _PI,
+                                       _PI, _PD, _PD, _PD]
+    L.ekxo_analytic_time3d.restype = _D
+    L.ekxo_analytic_time3d.argtypes = [_I, _I, _I, _D, _D, _D, _D, _D, _D, _D]
+    L.ekxo_analytic_time_deriv3d.restype = None
+    L.ekxo_analytic_time_deriv3d.argtypes = [_I, _I, _I, _D, _D, _D, _D, _D,
+                                             _D, _D, _PD, _PD, _PD, _PD]
+    L.ekxo_analytic_time2d.restype = _D
+    L.ekxo_analytic_time2d.argtypes = [_I, _I, _D, _D, _D, _D, _D]
+    L.ekxo_station_time2d.restype = _D
+    L.ekxo_station_time2d.argtypes = [ctypes.POINTER(Geometry2D), _PD, _PD,
+                                      _D, _D, _D, _D]
+    for name in ("ekxo_solve3d", "ekxo_solve3d_level_set", "ekxo_sweep3d",
+                 "ekxo_source3d", "ekxo_initialize_travel_times3d",
+                 "ekxo_homogeneous3d", "ekxo_linear_gradient3d",
+                 "ekxo_solve2d", "ekxo_sweep2d", "ekxo_source2d",
+                 "ekxo_homogeneous2d", "ekxo_linear_gradient2d",
+                 "ekxo_graph3d", "ekxo_nodal_to_cell3d",
+                 "ekxo_nodal_to_cell2d", "ekxo_cell_velocity_to_slowness3d"):
+        getattr(L, name).restype = _I
+
+
+def _dp(a):
+    return a.ctypes.data_as(_PD)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_PI)
+
+
+def geometry3d(nx, ny, nz, dx, dy=None, dz=None, x0=0.0, y0=0.0, z0=0.0):
+    dy = dx if dy is None else dy
+    dz = dx if dz is None else dz
+    return Geometry3D(nx, ny, nz, dx, dy, dz, x0, y0, z0)
+
+
+def geometry2d(nx, nz, dx, dz=None, x0=0.0, z0=0.0):
+    dz = dx if dz is None else dz
+    return Geometry2D(nx, nz, dx, dz, x0, z0)
+
+
+def n_cells3d(g):
+    return (g.nx - 1) * (g.ny - 1) * (g.nz - 1)
+
+
+def solve3d(g, slowness, source, n_sweeps, radius,
+            algorithm=ALGORITHM_FAST_SWEEPING, schedule=SCHEDULE_COMPLETED):
+    """Serial FSM restatement of Solver3D::solve (natural order output)."""
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    assert s.size == n_cells3d(g)
+    T = np.empty(g.nx * g.ny * g.nz, dtype=np.float64)
+    rc = lib().ekxo_solve3d(ctypes.byref(g), _dp(s), _D(source[0]),
+                            _D(source[1]), _D(source[2]), n_sweeps, radius,
+                            algorithm, schedule, _dp(T))
+    if rc != 0:
+        raise ValueError("oracle solve3d rejected its inputs")
+    return T
+
+
+def solve3d_level_set(g, slowness, source, n_sweeps, radius,
+                      schedule=SCHEDULE_COMPLETED, threads=0):
+    """Hyperplane-parallel (OpenMP) restatement; same schedule/results."""
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.empty(g.nx * g.ny * g.nz, dtype=np.float64)
+    rc = lib().ekxo_solve3d_level_set(ctypes.byref(g), _dp(s), _D(source[0]),
+                                      _D(source[1]), _D(source[2]), n_sweeps,
+                                      radius, schedule, threads, _dp(T))
+    if rc != 0:
+        raise ValueError("oracle solve3d_level_set rejected its inputs")
+    return T
+
+
+def sweep3d(g, slowness, source, radius, sweep, initialize, T):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    assert T.dtype == np.float64 and T.flags.c_contiguous
+    rc = lib().ekxo_sweep3d(ctypes.byref(g), _dp(s), _D(source[0]),
+                            _D(source[1]), _D(source[2]), radius, sweep,
+                            1 if initialize else 0, _dp(T))
+    if rc != 0:
+        raise ValueError("oracle sweep3d rejected its inputs")
+    return T
+
+
+def initialize_travel_times3d(g, slowness, source):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.empty(g.nx * g.ny * g.nz, dtype=np.float64)
+    rc = lib().ekxo_initialize_travel_times3d(
+        ctypes.byref(g), _dp(s), _D(source[0]), _D(source[1]),
+        _D(source[2]), _dp(T))
+    if rc != 0:
+        raise ValueError("source outside the model")
+    return T
+
+
+def source3d(g, x, y, z):
+    off = np.zeros(3)
+    cell = np.zeros(3, dtype=np.int32)
+    idx = ctypes.c_int(0)
+    rc = lib().ekxo_source3d(ctypes.byref(g), _D(x), _D(y), _D(z), _dp(off),
+                             _ip(cell), ctypes.byref(idx))
+    if rc != 0:
+        raise ValueError("source outside the model")
+    return off, cell, idx.value
+
+
+def source2d(g, x, z):
+    off = np.zeros(2)
+    cell = np.zeros(2, dtype=np.int32)
+    idx = ctypes.c_int(0)
+    rc = lib().ekxo_source2d(ctypes.byref(g), _D(x), _D(z), _dp(off),
+                             _ip(cell), ctypes.byref(idx))
+    if rc != 0:
+        raise ValueError("source outside the model")
+    return off, cell, idx.value
+
+
+def nodal_to_cell3d(nx, ny, nz, v, ordering=0):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    s = np.empty((nx - 1) * (ny - 1) * (nz - 1))
+    if lib().ekxo_nodal_to_cell3d(nx, ny, nz, _dp(v), ordering, _dp(s)) != 0:
+        raise ValueError("non-positive velocities")
+    return s
+
+
+def cell_velocity_to_slowness3d(nx, ny, nz, v, ordering=0):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    s = np.empty((nx - 1) * (ny - 1) * (nz - 1))
+    if lib().ekxo_cell_velocity_to_slowness3d(nx, ny, nz, _dp(v), ordering,
+                                              _dp(s)) != 0:
+        raise ValueError("non-positive velocities")
+    return s
+
+
+def nodal_to_cell2d(nx, nz, v, ordering=0):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    s = np.empty((nx - 1) * (nz - 1))
+    if lib().ekxo_nodal_to_cell2d(nx, nz, _dp(v), ordering, _dp(s)) != 0:
+        raise ValueError("non-positive velocities")
+    return s
+
+
+def homogeneous3d(g, source, vel):
+    T = np.empty(g.nx * g.ny * g.nz)
+    lib().ekxo_homogeneous3d(ctypes.byref(g), _D(source[0]), _D(source[1]),
+                             _D(source[2]), _D(vel), _dp(T))
+    return T
+
+
+def linear_gradient3d(g, source, v_top, v_bottom):
+    T = np.empty(g.nx * g.ny * g.nz)
+    lib().ekxo_linear_gradient3d(ctypes.byref(g), _D(source[0]),
+                                 _D(source[1]), _D(source[2]), _D(v_top),
+                                 _D(v_bottom), _dp(T))
+    return T
+
+
+def solve2d(g, slowness, source, n_sweeps, radius):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.empty(g.nx * g.nz)
+    rc = lib().ekxo_solve2d(ctypes.byref(g), _dp(s), _D(source[0]),
+                            _D(source[1]), n_sweeps, radius, _dp(T))
+    if rc != 0:
+        raise ValueError("oracle solve2d rejected its inputs")
+    return T
+
+
+def sweep2d(g, slowness, source, radius, sweep, initialize, T):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    rc = lib().ekxo_sweep2d(ctypes.byref(g), _dp(s), _D(source[0]),
+                            _D(source[1]), radius, sweep,
+                            1 if initialize else 0, _dp(T))
+    if rc != 0:
+        raise ValueError("oracle sweep2d rejected its inputs")
+    return T
+
+
+def homogeneous2d(g, source, vel):
+    T = np.empty(g.nx * g.nz)
+    lib().ekxo_homogeneous2d(ctypes.byref(g), _D(source[0]), _D(source[1]),
+                             _D(vel), _dp(T))
+    return T
+
+
+def linear_gradient2d(g, source, v_top, v_bottom):
+    T = np.empty(g.nx * g.nz)
+    lib().ekxo_linear_gradient2d(ctypes.byref(g), _D(source[0]),
+                                 _D(source[1]), _D(v_top), _D(v_bottom),
+                                 _dp(T))
+    return T
+
+
+def station_time2d(g, slowness, T, source, receiver):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    return lib().ekxo_station_time2d(ctypes.byref(g), _dp(s), _dp(T),
+                                     _D(source[0]), _D(source[1]),
+                                     _D(receiver[0]), _D(receiver[1]))
+
+
+def graph3d(sweep, nx, ny, nz):
+    n_levels = nx + ny + nz - 2
+    start = np.zeros(n_levels + 1, dtype=np.int32)
+    nodes = np.zeros(nx * ny * nz, dtype=np.int32)
+    if lib().ekxo_graph3d(sweep, nx, ny, nz, _ip(start), _ip(nodes)) != 0:
+        raise ValueError("grid too small")
+    return start, nodes
+
+
+def _ints(n):
+    return [ctypes.c_int(0) for _ in range(n)]
+
+
+def sweep_signs3d(sweep):
+    v = _ints(6)
+    lib().ekxo_sweep_signs3d(sweep, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)  # ixShift,iyShift,izShift,sx,sy,sz
+
+
+def loop_limits3d(sweep, nx, ny, nz, init_source=None):
+    v = _ints(9)
+    refs = [ctypes.byref(x) for x in v]
+    if init_source is None:
+        lib().ekxo_loop_limits3d(sweep, nx, ny, nz, *refs)
+    else:
+        lib().ekxo_loop_limits_init3d(sweep, nx, ny, nz, *init_source, *refs)
+    return tuple(x.value for x in v)  # ix0,iy0,iz0,ix1,iy1,iz1,dirs
+
+
+def permute3d(sweep, ix, iy, iz, nx, ny, nz):
+    v = _ints(3)
+    lib().ekxo_permute3d(sweep, ix, iy, iz, nx, ny, nz,
+                         *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def slowness_indices3d(sweep, ix, iy, iz, ncx, ncy, ncz):
+    out = np.zeros(7, dtype=np.int32)
+    lib().ekxo_slowness_indices3d(sweep, ix, iy, iz, ncx, ncy, ncz, _ip(out))
+    return out
+
+
+def travel_time_indices3d(sweep, ix, iy, iz, nx, ny, nz):
+    out = np.zeros(8, dtype=np.int32)
+    lib().ekxo_travel_time_indices3d(sweep, ix, iy, iz, nx, ny, nz, _ip(out))
+    return out
+
+
+def is_sweep_boundary3d(sweep, ix, iy, iz, nx, ny, nz):
+    return bool(lib().ekxo_is_sweep_boundary3d(sweep, ix, iy, iz, nx, ny, nz))
+
+
+def fd3d_uniform(radius, h, s_src, ijk, signs, shifts, isrc, src_off, s7,
+                 t7, huge=1.0e10):
+    sg = (ctypes.c_int * 3)(*signs)
+    sh = (ctypes.c_int * 3)(*shifts)
+    isr = (ctypes.c_int * 3)(*isrc)
+    off = (ctypes.c_double * 3)(*src_off)
+    s = (ctypes.c_double * 7)(*s7)
+    t = (ctypes.c_double * 7)(*t7)
+    return lib().ekxo_fd3d_uniform(radius, huge, h, s_src, ijk[0], ijk[1],
+                                   ijk[2], sg, sh, isr, off, s, t)
+
+
+def fd3d_nonuniform(radius, dxyz, s_src, ijk, signs, shifts, isrc, src_off,
+                    s7, t7, huge=1.0e10):
+    sg = (ctypes.c_int * 3)(*signs)
+    sh = (ctypes.c_int * 3)(*shifts)
+    isr = (ctypes.c_int * 3)(*isrc)
+    off = (ctypes.c_double * 3)(*src_off)
+    s = (ctypes.c_double * 7)(*s7)
+    t = (ctypes.c_double * 7)(*t7)
+    return lib().ekxo_fd3d_nonuniform(radius, huge, dxyz[0], dxyz[1],
+                                      dxyz[2], s_src, ijk[0], ijk[1], ijk[2],
+                                      sg, sh, isr, off, s, t)
+
+
+def fd2d_uniform(radius, h, s_src, ik, signs, shifts, isrc, src_off, s3, t3,
+                 huge=1.0e10):
+    sg = (ctypes.c_int * 2)(*signs)
+    sh = (ctypes.c_int * 2)(*shifts)
+    isr = (ctypes.c_int * 2)(*isrc)
+    off = (ctypes.c_double * 2)(*src_off)
+    s = (ctypes.c_double * 3)(*s3)
+    t = (ctypes.c_double * 3)(*t3)
+    return lib().ekxo_fd2d_uniform(radius, huge, h, s_src, ik[0], ik[1], sg,
+                                   sh, isr, off, s, t)
+
+
+def sweep_signs2d(sweep):
+    v = _ints(4)
+    lib().ekxo_sweep_signs2d(sweep, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)  # ixShift, izShift, signX, signZ
+
+
+def loop_limits2d(sweep, nx, nz, init_source=None):
+    v = _ints(6)
+    refs = [ctypes.byref(x) for x in v]
+    if init_source is None:
+        lib().ekxo_loop_limits2d(sweep, nx, nz, *refs)
+    else:
+        lib().ekxo_loop_limits_init2d(sweep, init_source[0], init_source[1],
+                                      nx, nz, *refs)
+    return tuple(x.value for x in v)  # ix0,iz0,ix1,iz1,ixDir,izDir
+
+
+def slowness_indices2d(sweep, ix, iz, ncx, ncz):
+    out = np.zeros(3, dtype=np.int32)
+    lib().ekxo_slowness_indices2d(sweep, ix, iz, ncx, ncz, _ip(out))
+    return out
+
+
+def level_start_stop2d(nx, nz, level):
+    v = _ints(2)
+    lib().ekxo_level_start_stop2d(nx, nz, level, *[ctypes.byref(x) for x in v])
+    return v[0].value, v[1].value
+
+
+def level_travel_time_indices2d(sweep, level, indx, nx, nz):
+    out = np.zeros(4, dtype=np.int32)
+    lib().ekxo_level_travel_time_indices2d(sweep, level, indx, nx, nz,
+                                           _ip(out))
+    return out
+
+
+def grid_to_level_index2d(sweep, ix, iz, nx, nz):
+    v = _ints(2)
+    lib().ekxo_grid_to_level_index2d(sweep, ix, iz, nx, nz,
+                                     *[ctypes.byref(x) for x in v])
+    return v[0].value, v[1].value

@@ -1,0 +1,1647 @@
+/*
+ * ekx_oracle.c -- CPU restatement of the reference eikonal arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Not linked into,
+ * loaded by or called from the product library.  Build with
+ * -ffp-contract=off (see oracle/Makefile): expression order follows the
+ * reference source so that results are reproducible bit for bit.
+ *
+ * Pinning status of each group is listed in ekx_oracle.h.
+ */
+#include "ekx_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* include/private/grid.hpp:27-34 -- x fastest, then y, then z. */
+static inline int64_t grid3(int nx, int ny, int ix, int iy, int iz)
+{
+    return ((int64_t) iz * ny + iy) * nx + ix;
+}
+
+/* ======================================================================== */
+/*                       3D sweep index helpers                             */
+/* ======================================================================== */
+
+/* solverUtilities3d.hpp:843-925.  Sweep k (1..8): bit0 -> x decreasing,
+ * bit1 -> y decreasing, bit2 -> z decreasing (Sweep1 = +x+y+z,
+ * Sweep2 = -x+y+z, ..., Sweep8 = -x-y-z). */
+void ekxo_sweep_signs3d(int sweep, int *ixShift, int *iyShift, int *izShift,
+                        int *signX, int *signY, int *signZ)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negY = (k >> 1) & 1, negZ = (k >> 2) & 1;
+    *ixShift = negX ? +1 : -1;
+    *iyShift = negY ? +1 : -1;
+    *izShift = negZ ? +1 : -1;
+    *signX = negX ? -1 : 1;
+    *signY = negY ? -1 : 1;
+    *signZ = negZ ? -1 : 1;
+}
+
+/* solverUtilities3d.hpp:944-1052 (refinement sweeps). */
+void ekxo_loop_limits3d(int sweep, int nx, int ny, int nz,
+                        int *ix0, int *iy0, int *iz0,
+                        int *ix1, int *iy1, int *iz1,
+                        int *ixDir, int *iyDir, int *izDir)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negY = (k >> 1) & 1, negZ = (k >> 2) & 1;
+    if (!negX) { *ix0 = 1;      *ix1 = nx; *ixDir = 1; }
+    else       { *ix0 = nx - 2; *ix1 = -1; *ixDir = -1; }
+    if (!negY) { *iy0 = 1;      *iy1 = ny; *iyDir = 1; }
+    else       { *iy0 = ny - 2; *iy1 = -1; *iyDir = -1; }
+    if (!negZ) { *iz0 = 1;      *iz1 = nz; *izDir = 1; }
+    else       { *iz0 = nz - 2; *iz1 = -1; *izDir = -1; }
+}
+
+/* solverUtilities3d.hpp:1075-1184 (initialization sweeps: source octant). */
+void ekxo_loop_limits_init3d(int sweep, int nx, int ny, int nz,
+                             int iSrcX, int iSrcY, int iSrcZ,
+                             int *ix0, int *iy0, int *iz0,
+                             int *ix1, int *iy1, int *iz1,
+                             int *ixDir, int *iyDir, int *izDir)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negY = (k >> 1) & 1, negZ = (k >> 2) & 1;
+    if (!negX) { *ix0 = imax(1, iSrcX);          *ix1 = nx; *ixDir = 1; }
+    else       { *ix0 = imin(iSrcX + 1, nx - 2); *ix1 = -1; *ixDir = -1; }
+    if (!negY) { *iy0 = imax(1, iSrcY);          *iy1 = ny; *iyDir = 1; }
+    else       { *iy0 = imin(iSrcY + 1, ny - 2); *iy1 = -1; *iyDir = -1; }
+    if (!negZ) { *iz0 = imax(1, iSrcZ);          *iz1 = nz; *izDir = 1; }
+    else       { *iz0 = imin(iSrcZ + 1, nz - 2); *iz1 = -1; *izDir = -1; }
+}
+
+/* solverUtilities3d.hpp:1197-1257. */
+void ekxo_permute3d(int sweep, int ix, int iy, int iz, int nx, int ny, int nz,
+                    int *jx, int *jy, int *jz)
+{
+    int k = sweep - 1;
+    *jx = (k & 1) ? nx - 1 - ix : ix;
+    *jy = ((k >> 1) & 1) ? ny - 1 - iy : iy;
+    *jz = ((k >> 2) & 1) ? nz - 1 - iz : iz;
+}
+
+/* solverUtilities3d.hpp:1300-1442.  iCell7 = {c0,c1,c2,c3,c4,c5,c7}. */
+void ekxo_slowness_indices3d(int sweep, int ix, int iy, int iz,
+                             int nCellX, int nCellY, int nCellZ, int *iCell7)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negY = (k >> 1) & 1, negZ = (k >> 2) & 1;
+    int ixU, ixO, iyU, iyO, izU, izO;
+    if (!negX)
+    {
+        ixU = imax(0, imin(nCellX - 1, ix - 1));
+        ixO = imax(0, imin(nCellX - 1, ixU + 1));
+    }
+    else
+    {
+        ixU = imax(0, imin(nCellX - 1, ix));
+        ixO = imax(0, imin(nCellX - 1, ixU - 1));
+    }
+    if (!negY)
+    {
+        iyU = imax(0, imin(nCellY - 1, iy - 1));
+        iyO = imax(0, imin(nCellY - 1, iyU + 1));
+    }
+    else
+    {
+        iyU = imax(0, imin(nCellY - 1, iy));
+        iyO = imax(0, imin(nCellY - 1, iyU - 1));
+    }
+    if (!negZ)
+    {
+        izU = imax(0, imin(nCellZ - 1, iz - 1));
+        izO = imax(0, imin(nCellZ - 1, izU + 1));
+    }
+    else
+    {
+        izU = imax(0, imin(nCellZ - 1, iz));
+        izO = imax(0, imin(nCellZ - 1, izU - 1));
+    }
+    iCell7[0] = (int) grid3(nCellX, nCellY, ixU, iyU, izU);
+    iCell7[1] = (int) grid3(nCellX, nCellY, ixO, iyU, izU);
+    iCell7[2] = (int) grid3(nCellX, nCellY, ixO, iyO, izU);
+    iCell7[3] = (int) grid3(nCellX, nCellY, ixU, iyO, izU);
+    iCell7[4] = (int) grid3(nCellX, nCellY, ixU, iyU, izO);
+    iCell7[5] = (int) grid3(nCellX, nCellY, ixO, iyU, izO);
+    iCell7[6] = (int) grid3(nCellX, nCellY, ixU, iyO, izO);
+}
+
+/* solverUtilities3d.hpp:1460-1586.  it8 = {i0..i7}. */
+void ekxo_travel_time_indices3d(int sweep, int ix, int iy, int iz,
+                                int nx, int ny, int nz, int *it8)
+{
+    int k = sweep - 1;
+    int sx = (k & 1) ? 1 : -1;
+    int sy = ((k >> 1) & 1) ? 1 : -1;
+    int sz = ((k >> 2) & 1) ? 1 : -1;
+    int ixS = imax(0, imin(nx - 1, ix + sx));
+    int iyS = imax(0, imin(ny - 1, iy + sy));
+    int izS = imax(0, imin(nz - 1, iz + sz));
+    it8[0] = (int) grid3(nx, ny, ix,  iy,  iz);
+    it8[1] = (int) grid3(nx, ny, ixS, iy,  iz);
+    it8[2] = (int) grid3(nx, ny, ixS, iyS, iz);
+    it8[3] = (int) grid3(nx, ny, ix,  iyS, iz);
+    it8[4] = (int) grid3(nx, ny, ix,  iy,  izS);
+    it8[5] = (int) grid3(nx, ny, ixS, iy,  izS);
+    it8[6] = (int) grid3(nx, ny, ixS, iyS, izS);
+    it8[7] = (int) grid3(nx, ny, ix,  iyS, izS);
+}
+
+/* solverUtilities3d.hpp:1598-1643. */
+int ekxo_is_sweep_boundary3d(int sweep, int ix, int iy, int iz,
+                             int nx, int ny, int nz)
+{
+    int k = sweep - 1;
+    int bx = (k & 1) ? (ix == nx - 1) : (ix == 0);
+    int by = ((k >> 1) & 1) ? (iy == ny - 1) : (iy == 0);
+    int bz = ((k >> 2) & 1) ? (iz == nz - 1) : (iz == 0);
+    return (bx || by || bz) ? 1 : 0;
+}
+
+/* src/graph3d.cpp:324-329. */
+int ekxo_graph3d_number_of_levels(int nx, int ny, int nz)
+{
+    return nx + ny + nz - 2;
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *) a, y = *(const int *) b;
+    return (x > y) - (x < y);
+}
+
+/* src/graph3d.cpp:119-221: nodes of level L = jx+jy+jz in the permuted
+ * frame, stored as natural grid indices sorted ascending in each level. */
+int ekxo_graph3d(int sweep, int nx, int ny, int nz,
+                 int *levelStart, int *nodeToIndex)
+{
+    if (nx < 3 || ny < 3 || nz < 3) { return -1; }
+    int nLevels = ekxo_graph3d_number_of_levels(nx, ny, nz);
+    levelStart[0] = 0;
+    for (int level = 0; level < nLevels; ++level)
+    {
+        int k1 = imax(0, level - (nx - 1) - (ny - 1));
+        k1 = imin(k1, nz - 1);
+        int k2 = imin(nz - 1, level);
+        int n = 0;
+        int offset = levelStart[level];
+        for (int iz = k1; iz <= k2; ++iz)
+        {
+            int j1 = imax(0, level - iz - (nx - 1));
+            j1 = imin(j1, ny - 1);
+            int j2 = imin(ny - 1, level - iz);
+            for (int iy = j1; iy <= j2; ++iy)
+            {
+                int i1 = imax(0, level - iz - iy);
+                i1 = imin(i1, nx - 1);
+                int i2 = imin(nx - 1, level - iz - iy);
+                for (int ix = i1; ix <= i2; ++ix)
+                {
+                    int jx, jy, jz;
+                    ekxo_permute3d(sweep, ix, iy, iz, nx, ny, nz,
+                                   &jx, &jy, &jz);
+                    nodeToIndex[offset + n] = (int) grid3(nx, ny, jx, jy, jz);
+                    n = n + 1;
+                }
+            }
+        }
+        levelStart[level + 1] = levelStart[level] + n;
+        qsort(nodeToIndex + offset, (size_t) n, sizeof(int), cmp_int);
+    }
+    return 0;
+}
+
+/* ======================================================================== */
+/*                    3D analytic times and local solvers                   */
+/* ======================================================================== */
+
+/* solverUtilities3d.hpp:138-156. */
+double ekxo_analytic_time3d(int ix, int iy, int iz,
+                            double dx, double dy, double dz,
+                            double xs, double ys, double zs, double slow)
+{
+    double x = ix*dx;
+    double y = iy*dy;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaY = y - ys;
+    double deltaZ = z - zs;
+    double distance = sqrt(deltaX*deltaX + deltaY*deltaY + deltaZ*deltaZ);
+    return slow*distance;
+}
+
+/* solverUtilities3d.hpp:180-208. */
+void ekxo_analytic_time_deriv3d(int ix, int iy, int iz,
+                                double dx, double dy, double dz,
+                                double xs, double ys, double zs, double slow,
+                                double *t, double *dtdx, double *dtdy,
+                                double *dtdz)
+{
+    double x = ix*dx;
+    double y = iy*dy;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaY = y - ys;
+    double deltaZ = z - zs;
+    double distance = sqrt(deltaX*deltaX + deltaY*deltaY + deltaZ*deltaZ);
+    *t = slow*distance;
+    *dtdx = 0;
+    *dtdy = 0;
+    *dtdz = 0;
+    if (distance > 0)
+    {
+        *dtdx = (slow*deltaX)/distance;
+        *dtdy = (slow*deltaY)/distance;
+        *dtdz = (slow*deltaZ)/distance;
+    }
+}
+
+/* solverUtilities3d.hpp:258-558 (uniform grid, FTeik form). */
+double ekxo_fd3d_uniform(int radius, double huge, double h, double sSrc,
+                         int ix, int iy, int iz, const int *signs,
+                         const int *shifts, const int *iSrc,
+                         const double *srcOffset, const double *s7,
+                         const double *t7)
+{
+    const double s0 = s7[0], s1 = s7[1], s2 = s7[2], s3 = s7[3];
+    const double s4 = s7[4], s5 = s7[5], s7v = s7[6];
+    const double t1 = t7[0], t2 = t7[1], t3 = t7[2], t4 = t7[3];
+    const double t5 = t7[4], t6 = t7[5], t7v = t7[6];
+    double tUpd = huge;
+    double minS0S3 = fmin(s0, s3);
+    double minS0S1 = fmin(s0, s1);
+    double minS0S4 = fmin(s0, s4);
+    double t1d1 = t1 + h*fmin(minS0S3, fmin(s4, s7v));
+    double t1d2 = t3 + h*fmin(minS0S4, fmin(s1, s5));
+    double t1d3 = t4 + h*fmin(minS0S1, fmin(s2, s3));
+    double t1d = fmin(fmin(t1d1, t1d2), t1d3);
+    double hs03 = h*minS0S3;
+    double hs01 = h*minS0S1;
+    double hs04 = h*minS0S4;
+    if (abs(iSrc[0] - ix) > radius ||
+        abs(iSrc[1] - iy) > radius ||
+        abs(iSrc[2] - iz) > radius)
+    {
+        double dtCrossXZ = t1 - t4;
+        double detXZ = 2*(hs03*hs03) - dtCrossXZ*dtCrossXZ;
+        double t2d1 = huge;
+        if (detXZ > 0 && t4 <= t1 + hs03 && t1 <= t4 + hs03)
+        {
+            t2d1 = t5 + sqrt(detXZ);
+        }
+        double t2d = t2d1;
+        double dtCrossYZ = t3 - t4;
+        double detYZ = 2*(hs01*hs01) - dtCrossYZ*dtCrossYZ;
+        double t2d2 = huge;
+        if (detYZ > 0 && t3 <= t4 + hs01 && t4 <= t3 + hs01)
+        {
+            t2d2 = t7v + sqrt(detYZ);
+        }
+        t2d = fmin(t2d, t2d2);
+        double dtCrossXY = t1 - t3;
+        double detXY = 2*(hs04*hs04) - dtCrossXY*dtCrossXY;
+        double t2d3 = huge;
+        if (detXY > 0 && t3 <= t1 + hs04 && t1 <= t3 + hs04)
+        {
+            t2d3 = t2 + sqrt(detXY);
+        }
+        t2d = fmin(t2d, t2d3);
+        double t3d = huge;
+        if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+        {
+            const double third = 0.33333333333333333333333;
+            const double half = 0.5;
+            double Tx = t1 + half*(-t4 + t5 - t3 + t2) - t7v + t6;
+            double Ty = t3 + half*(-t4 + t7v - t1 + t2) - t5 + t6;
+            double Tz = t4 + half*(-t1 + t5 - t3 + t7v) - t2 + t6;
+            double det = 27*(h*s0)*(h*s0)
+                       - (Tx - Ty)*(Tx - Ty)
+                       - (Tx - Tz)*(Tx - Tz)
+                       - (Ty - Tz)*(Ty - Tz);
+            if (det >= 0)
+            {
+                t3d = third*((Tx + Ty + Tz) + sqrt(det));
+            }
+        }
+        tUpd = fmin(t1d, fmin(t2d, t3d));
+    }
+    else
+    {
+        const double dx = h, dy = h, dz = h;
+        const double xs = srcOffset[0], ys = srcOffset[1], zs = srcOffset[2];
+        const int ixShift = shifts[0], iyShift = shifts[1], izShift = shifts[2];
+        double t0, dt0dx, dt0dy, dt0dz;
+        ekxo_analytic_time_deriv3d(ix, iy, iz, dx, dy, dz, xs, ys, zs, s0,
+                                   &t0, &dt0dx, &dt0dy, &dt0dz);
+        double tau1 = t1 - ekxo_analytic_time3d(ix + ixShift, iy, iz,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau2 = t2 - ekxo_analytic_time3d(ix + ixShift, iy + iyShift,
+                                                iz, dx, dy, dz, xs, ys, zs,
+                                                s0);
+        double tau3 = t3 - ekxo_analytic_time3d(ix, iy + iyShift, iz,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau4 = t4 - ekxo_analytic_time3d(ix, iy, iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau5 = t5 - ekxo_analytic_time3d(ix + ixShift, iy,
+                                                iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau6 = t6 - ekxo_analytic_time3d(ix + ixShift, iy + iyShift,
+                                                iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau7 = t7v - ekxo_analytic_time3d(ix, iy + iyShift,
+                                                 iz + izShift,
+                                                 dx, dy, dz, xs, ys, zs, s0);
+        dt0dx = signs[0]*dt0dx;
+        dt0dy = signs[1]*dt0dy;
+        dt0dz = signs[2]*dt0dz;
+        double dTauTxXZ = tau1 - tau4 + tau5;
+        double dTauTzXZ = tau4 - tau1 + tau5;
+        const double half = 0.5;
+        double hi = 1/h;
+        double h2inv = 1/(h*h);
+        double a = h2inv;
+        double b = (2*hi)*(dt0dx + dt0dz)
+                 - (2*h2inv)*tau5;
+        double c = (half*h2inv)*(dTauTxXZ*dTauTxXZ + dTauTzXZ*dTauTzXZ)
+                 - (2*hi)*(dt0dx*dTauTxXZ + dt0dz*dTauTzXZ)
+                 + 2*(sSrc*sSrc - s0*s0 + dt0dy*dt0dy);
+        double detXZ = b*b - 4*a*c;
+        double tau = 0;
+        double t2d1 = huge;
+        if (detXZ > 0 && t4 <= t1 + hs03 && t1 <= t4 + hs03)
+        {
+            tau = (-b + sqrt(detXZ))/(2*a);
+            t2d1 = t0 + tau;
+            if (t2d1 < t1 || t2d1 < t4) { t2d1 = huge; }
+        }
+        double t2d = t2d1;
+        double dTauTyYZ = tau3 - tau4 + tau7;
+        double dTauTzYZ = tau4 - tau3 + tau7;
+        b = (2*hi)*(dt0dy + dt0dz)
+          - (2*h2inv)*tau7;
+        c = (half*h2inv)*(dTauTyYZ*dTauTyYZ + dTauTzYZ*dTauTzYZ)
+          - (2*hi)*(dt0dy*dTauTyYZ + dt0dz*dTauTzYZ)
+          + 2*(sSrc*sSrc - s0*s0 + dt0dx*dt0dx);
+        double detYZ = b*b - 4*a*c;
+        double t2d2 = huge;
+        if (detYZ >= 0 && t3 <= t4 + hs01 && t4 <= t3 + hs01)
+        {
+            tau = (-b + sqrt(detYZ))/(2*a);
+            t2d2 = t0 + tau;
+            if (t2d2 < t3 || t2d2 < t4) { t2d2 = huge; }
+        }
+        t2d = fmin(t2d, t2d2);
+        double dTauTxXY = tau1 - tau3 + tau2;
+        double dTauTyXY = tau3 - tau1 + tau2;
+        b = (2*hi)*(dt0dx + dt0dy)
+          - (2*h2inv)*tau2;
+        c = (half*h2inv)*(dTauTxXY*dTauTxXY + dTauTyXY*dTauTyXY)
+          - (2*hi)*(dt0dx*dTauTxXY + dt0dy*dTauTyXY)
+          + 2*(sSrc*sSrc - s0*s0 + dt0dz*dt0dz);
+        double detXY = b*b - 4*a*c;
+        double t2d3 = huge;
+        if (detXY >= 0 && t3 <= t1 + hs04 && t1 <= t3 + hs04)
+        {
+            tau = (-b + sqrt(detXY))/(2*a);
+            t2d3 = t0 + tau;
+            if (t2d3 < t1 || t2d3 < t3) { t2d3 = huge; }
+        }
+        t2d = fmin(t2d, t2d3);
+        double t3d = huge;
+        if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+        {
+            double taux = tau1 + half*(-tau4 + tau5 - tau3 + tau2) - tau7 + tau6;
+            double tauy = tau3 + half*(-tau4 + tau7 - tau1 + tau2) - tau5 + tau6;
+            double tauz = tau4 + half*(-tau1 + tau5 - tau3 + tau7) - tau2 + tau6;
+            a = 3*h2inv;
+            b =-2*h2inv*(taux + tauy + tauz)
+              + (6*hi)*(dt0dx + dt0dy + dt0dz);
+            c = h2inv*(taux*taux + tauy*tauy + tauz*tauz)
+              - (6*hi)*(dt0dx*taux + dt0dy*tauy + dt0dz*tauz)
+              + 9*(sSrc*sSrc - s0*s0);
+            double det = b*b - 4*a*c;
+            if (det >= 0)
+            {
+                t3d = t0 + (-b + sqrt(det))/(2*a);
+            }
+            if (t3d < t1 || t3d < t3 || t3d < t4) { t3d = huge; }
+        }
+        tUpd = fmin(t1d, fmin(t2d, t3d));
+    }
+    return tUpd;
+}
+
+/* solverUtilities3d.hpp:562-824 (non-uniform grid).  The derived grid
+ * constants follow include/private/solver3d.hpp:247-261. */
+double ekxo_fd3d_nonuniform(int radius, double huge,
+                            double dx, double dy, double dz, double sSrc,
+                            int ix, int iy, int iz, const int *signs,
+                            const int *shifts, const int *iSrc,
+                            const double *srcOffset, const double *s7,
+                            const double *t7)
+{
+    double dxInv = 1/dx;
+    double dyInv = 1/dy;
+    double dzInv = 1/dz;
+    double dx2Inv = 1/(dx*dx);
+    double dy2Inv = 1/(dy*dy);
+    double dz2Inv = 1/(dz*dz);
+    double dx_dz = dx/dz;
+    double dz_dx = dz/dx;
+    double dy_dz = dy/dz;
+    double dz_dy = dz/dy;
+    double dx_dy = dx/dy;
+    double dy_dx = dy/dx;
+    double dx2_p_dz2_inv = 1/(dx*dx + dz*dz);
+    double dy2_p_dz2_inv = 1/(dy*dy + dz*dz);
+    double dx2_p_dy2_inv = 1/(dx*dx + dy*dy);
+
+    const double s0 = s7[0], s1 = s7[1], s2 = s7[2], s3 = s7[3];
+    const double s4 = s7[4], s5 = s7[5], s7v = s7[6];
+    const double t1 = t7[0], t2 = t7[1], t3 = t7[2], t4 = t7[3];
+    const double t5 = t7[4], t6 = t7[5], t7v = t7[6];
+    double tUpd = huge;
+    double minS0S3 = fmin(s0, s3);
+    double minS0S1 = fmin(s0, s1);
+    double minS0S4 = fmin(s0, s4);
+    double t1d1 = t1 + dx*fmin(minS0S3, fmin(s4, s7v));
+    double t1d2 = t3 + dy*fmin(minS0S4, fmin(s1, s5));
+    double t1d3 = t4 + dz*fmin(minS0S1, fmin(s2, s3));
+    double t1d = fmin(fmin(t1d1, t1d2), t1d3);
+    if (abs(iSrc[0] - ix) > radius ||
+        abs(iSrc[1] - iy) > radius ||
+        abs(iSrc[2] - iz) > radius)
+    {
+        double dtCrossXZ = t1 - t4;
+        double detXZ = (dx*dx + dz*dz)*(minS0S3*minS0S3) - dtCrossXZ*dtCrossXZ;
+        double t2d1 = huge;
+        if (detXZ > 0 && t1 < t4 + dz*minS0S3 && t4 < t1 + dx*minS0S3)
+        {
+            double TxXZ = t1 - t4 + t5;
+            double TzXZ = t4 - t1 + t5;
+            t2d1 = (TxXZ*dz_dx + TzXZ*dx_dz) + 2*sqrt(detXZ);
+            t2d1 = ((dx*dz)*dx2_p_dz2_inv)*t2d1;
+        }
+        double t2d = t2d1;
+        double dtCrossYZ = t3 - t4;
+        double detYZ = (dy*dy + dz*dz)*(minS0S1*minS0S1) - dtCrossYZ*dtCrossYZ;
+        double t2d2 = huge;
+        if (detYZ > 0 && t3 < t4 + dz*minS0S1 && t4 < t3 + dy*minS0S1)
+        {
+            double TyYZ = t3 - t4 + t7v;
+            double TzYZ = t4 - t3 + t7v;
+            t2d2 = (TyYZ*dz_dy + TzYZ*dy_dz) + 2*sqrt(detYZ);
+            t2d2 = ((dy*dz)*dy2_p_dz2_inv)*t2d2;
+        }
+        t2d = fmin(t2d, t2d2);
+        double dtCrossXY = t1 - t3;
+        double detXY = (dx*dx + dy*dy)*(minS0S4*minS0S4) - dtCrossXY*dtCrossXY;
+        double t2d3 = huge;
+        if (detXY > 0 && t3 < t1 + dx*minS0S4 && t1 < t3 + dy*minS0S4)
+        {
+            double TxXY = t1 - t3 + t2;
+            double TyXY = t3 - t1 + t2;
+            t2d3 = (TxXY*dy_dx + TyXY*dx_dy) + 2*sqrt(detXY);
+            t2d3 = ((dx*dy)*dx2_p_dy2_inv)*t2d3;
+        }
+        t2d = fmin(t2d, t2d3);
+        double t3d = huge;
+        if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+        {
+            const double half = 0.5;
+            double Tx = t1 + half*(-t4 + t5 - t3 + t2) - t7v + t6;
+            double Ty = t3 + half*(-t4 + t7v - t1 + t2) - t5 + t6;
+            double Tz = t4 + half*(-t1 + t5 - t3 + t7v) - t2 + t6;
+            double det = 9*(s0*s0)*(dx2Inv + dy2Inv + dz2Inv)
+                       - ((Tx - Tz)*(Tx - Tz))*(dx2Inv*dz2Inv)
+                       - ((Ty - Tz)*(Ty - Tz))*(dy2Inv*dz2Inv)
+                       - ((Tx - Ty)*(Tx - Ty))*(dx2Inv*dy2Inv);
+            if (det >= 0)
+            {
+                t3d = (Tx*dx2Inv + Ty*dy2Inv + Tz*dz2Inv + sqrt(det))
+                     /(dx2Inv + dy2Inv + dz2Inv);
+            }
+        }
+        tUpd = fmin(t1d, fmin(t2d, t3d));
+    }
+    else
+    {
+        const double xs = srcOffset[0], ys = srcOffset[1], zs = srcOffset[2];
+        const int ixShift = shifts[0], iyShift = shifts[1], izShift = shifts[2];
+        double t0, dt0dx, dt0dy, dt0dz;
+        ekxo_analytic_time_deriv3d(ix, iy, iz, dx, dy, dz, xs, ys, zs, s0,
+                                   &t0, &dt0dx, &dt0dy, &dt0dz);
+        double tau1 = t1 - ekxo_analytic_time3d(ix + ixShift, iy, iz,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau2 = t2 - ekxo_analytic_time3d(ix + ixShift, iy + iyShift,
+                                                iz, dx, dy, dz, xs, ys, zs,
+                                                s0);
+        double tau3 = t3 - ekxo_analytic_time3d(ix, iy + iyShift, iz,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau4 = t4 - ekxo_analytic_time3d(ix, iy, iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau5 = t5 - ekxo_analytic_time3d(ix + ixShift, iy,
+                                                iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau6 = t6 - ekxo_analytic_time3d(ix + ixShift, iy + iyShift,
+                                                iz + izShift,
+                                                dx, dy, dz, xs, ys, zs, s0);
+        double tau7 = t7v - ekxo_analytic_time3d(ix, iy + iyShift,
+                                                 iz + izShift,
+                                                 dx, dy, dz, xs, ys, zs, s0);
+        dt0dx = signs[0]*dt0dx;
+        dt0dy = signs[1]*dt0dy;
+        dt0dz = signs[2]*dt0dz;
+        double dTauTxXZ = tau1 - tau4 + tau5;
+        double dTauTzXZ = tau4 - tau1 + tau5;
+        double a = dx2Inv + dz2Inv;
+        double b = 4*(dxInv*dt0dx + dzInv*dt0dz)
+                 - 2*(dTauTxXZ*dx2Inv + dTauTzXZ*dz2Inv);
+        double c = (dTauTxXZ*dTauTxXZ)*dx2Inv + (dTauTzXZ*dTauTzXZ)*dz2Inv
+                 - 4*((dt0dx*dTauTxXZ)*dxInv + (dt0dz*dTauTzXZ)*dzInv)
+                 + 4*(sSrc*sSrc - s0*s0 + dt0dy*dt0dy);
+        double detXZ = b*b - 4*a*c;
+        double t2d1 = huge;
+        if (detXZ > 0 && t1 < t4 + dz*minS0S3 && t4 < t1 + dx*minS0S3)
+        {
+            t2d1 = t0 + (-b + sqrt(detXZ))/(2*a);
+            if (t2d1 < t1 || t2d1 < t4) { t2d1 = huge; }
+        }
+        double t2d = t2d1;
+        double dTauTyYZ = tau3 - tau4 + tau7;
+        double dTauTzYZ = tau4 - tau3 + tau7;
+        a = dy2Inv + dz2Inv;
+        b = 4*(dyInv*dt0dy + dzInv*dt0dz)
+          - 2*(dTauTyYZ*dy2Inv + dTauTzYZ*dz2Inv);
+        c = (dTauTyYZ*dTauTyYZ)*dy2Inv + (dTauTzYZ*dTauTzYZ)*dz2Inv
+          - 4*((dt0dy*dTauTyYZ)*dyInv + (dt0dz*dTauTzYZ)*dzInv)
+          + 4*(sSrc*sSrc - s0*s0 + dt0dx*dt0dx);
+        double detYZ = b*b - 4*a*c;
+        double t2d2 = huge;
+        if (detYZ > 0 && t3 < t4 + dz*minS0S1 && t4 < t3 + dy*minS0S1)
+        {
+            t2d2 = t0 + (-b + sqrt(detYZ))/(2*a);
+            if (t2d2 < t3 || t2d2 < t4) { t2d2 = huge; }
+        }
+        t2d = fmin(t2d, t2d2);
+        double dTauTxXY = tau1 - tau3 + tau2;
+        double dTauTyXY = tau3 - tau1 + tau2;
+        a = dx2Inv + dy2Inv;
+        b = 4*(dxInv*dt0dx + dyInv*dt0dy)
+          - 2*(dTauTxXY*dx2Inv + dTauTyXY*dy2Inv);
+        c = (dTauTxXY*dTauTxXY)*dx2Inv + (dTauTyXY*dTauTyXY)*dy2Inv
+          - 4*((dt0dx*dTauTxXY)*dxInv + (dt0dy*dTauTyXY)*dyInv)
+          + 4*(sSrc*sSrc - s0*s0 + dt0dz*dt0dz);
+        double detXY = b*b - 4*a*c;
+        double t2d3 = huge;
+        if (detXY > 0 && t3 < t1 + dx*minS0S4 && t1 < t3 + dy*minS0S4)
+        {
+            t2d3 = t0 + (-b + sqrt(detXY))/(2*a);
+            if (t2d3 < t1 || t2d3 < t3) { t2d3 = huge; }
+        }
+        t2d = fmin(t2d, t2d3);
+        double t3d = huge;
+        if (fmin(t1d, t2d) > fmax(fmax(t1, t4), t3))
+        {
+            const double half = 0.5;
+            double taux = tau1 + half*(-tau4 + tau5 - tau3 + tau2) - tau7 + tau6;
+            double tauy = tau3 + half*(-tau4 + tau7 - tau1 + tau2) - tau5 + tau6;
+            double tauz = tau4 + half*(-tau1 + tau5 - tau3 + tau7) - tau2 + tau6;
+            a = dx2Inv + dy2Inv + dz2Inv;
+            b =-2*(taux*dx2Inv + tauy*dy2Inv + tauz*dz2Inv)
+              + 6*(dxInv*dt0dx + dyInv*dt0dy + dzInv*dt0dz);
+            c = (taux*taux)*dx2Inv + (tauy*tauy)*dy2Inv + (tauz*tauz)*dz2Inv
+              - 6*(dt0dx*taux*dxInv + dt0dy*tauy*dyInv + dt0dz*tauz*dzInv)
+              + 9*(sSrc*sSrc - s0*s0);
+            double det = b*b - 4*a*c;
+            if (det >= 0)
+            {
+                t3d = t0 + (-b + sqrt(det))/(2*a);
+            }
+            if (t3d < t1 || t3d < t3 || t3d < t4) { t3d = huge; }
+        }
+        tUpd = fmin(t1d, fmin(t2d, t3d));
+    }
+    return tUpd;
+}
+
+/* ======================================================================== */
+/*                        3D source, model, solver                          */
+/* ======================================================================== */
+
+/* src/source3d.cpp:161-181, 216-236, 271-291 and updateCell :11-22. */
+int ekxo_source3d(const ekxo_geometry3d *g, double x, double y, double z,
+                  double *offset3, int *cell3, int *cellIndex)
+{
+    int nCellX = g->nx - 1, nCellY = g->ny - 1, nCellZ = g->nz - 1;
+    double x1 = g->x0 + (double) (g->nx - 1)*g->dx;
+    double y1 = g->y0 + (double) (g->ny - 1)*g->dy;
+    double z1 = g->z0 + (double) (g->nz - 1)*g->dz;
+    if (x < g->x0 || x > x1) { return -1; }
+    if (y < g->y0 || y > y1) { return -1; }
+    if (z < g->z0 || z > z1) { return -1; }
+    offset3[0] = x - g->x0;
+    offset3[1] = y - g->y0;
+    offset3[2] = z - g->z0;
+    cell3[0] = imin(nCellX - 1, (int) (offset3[0]/g->dx));
+    cell3[1] = imin(nCellY - 1, (int) (offset3[1]/g->dy));
+    cell3[2] = imin(nCellZ - 1, (int) (offset3[2]/g->dz));
+    *cellIndex = (int) grid3(nCellX, nCellY, cell3[0], cell3[1], cell3[2]);
+    return 0;
+}
+
+/* src/model3d.cpp:45-151 (interpolate3d) and :285-314. */
+int ekxo_nodal_to_cell3d(int nx, int ny, int nz, const double *v,
+                         int ordering, double *slow)
+{
+    size_t nGrid = (size_t) nx*ny*nz;
+    for (size_t i = 0; i < nGrid; ++i) { if (!(v[i] > 0)) { return -1; } }
+    size_t nCellX = (size_t) nx - 1, nCellY = (size_t) ny - 1;
+    size_t nCellZ = (size_t) nz - 1;
+    const double eight = 8;
+    for (size_t iz = 0; iz < nCellZ; ++iz)
+    {
+        for (size_t iy = 0; iy < nCellY; ++iy)
+        {
+            for (size_t ix = 0; ix < nCellX; ++ix)
+            {
+                size_t idst = (iz*nCellY + iy)*nCellX + ix;
+                size_t idx1, idx2, idx3, idx4, idx5, idx6, idx7, idx8;
+                if (ordering == 0)
+                {
+                    idx1 = iz*(nCellX + 1)*(nCellY + 1) + iy*(nCellX + 1) + ix;
+                    idx2 = idx1 + 1;
+                    idx3 = idx1 + (nCellX + 1);
+                    idx4 = idx3 + 1;
+                    idx5 = idx1 + (nCellX + 1)*(nCellY + 1);
+                    idx6 = idx2 + (nCellX + 1)*(nCellY + 1);
+                    idx7 = idx3 + (nCellX + 1)*(nCellY + 1);
+                    idx8 = idx4 + (nCellX + 1)*(nCellY + 1);
+                }
+                else
+                {
+                    idx1 = ix*(nCellZ + 1)*(nCellY + 1) + iy*(nCellZ + 1) + iz;
+                    idx2 = idx1 + 1;
+                    idx3 = idx1 + (nCellZ + 1);
+                    idx4 = idx3 + 1;
+                    idx5 = idx1 + (nCellZ + 1)*(nCellY + 1);
+                    idx6 = idx2 + (nCellZ + 1)*(nCellY + 1);
+                    idx7 = idx3 + (nCellZ + 1)*(nCellY + 1);
+                    idx8 = idx4 + (nCellZ + 1)*(nCellY + 1);
+                }
+                double sum1 = v[idx1] + v[idx2];
+                double sum2 = v[idx3] + v[idx4];
+                double sum3 = v[idx5] + v[idx6];
+                double sum4 = v[idx7] + v[idx8];
+                slow[idst] = eight/(sum1 + sum2 + sum3 + sum4);
+            }
+        }
+    }
+    return 0;
+}
+
+/* src/model3d.cpp:316-375. */
+int ekxo_cell_velocity_to_slowness3d(int nx, int ny, int nz, const double *v,
+                                     int ordering, double *slow)
+{
+    int nCellX = nx - 1, nCellY = ny - 1, nCellZ = nz - 1;
+    size_t nCell = (size_t) nCellX*nCellY*nCellZ;
+    for (size_t i = 0; i < nCell; ++i) { if (!(v[i] > 0)) { return -1; } }
+    const double one = 1;
+    if (ordering == 0)
+    {
+        for (size_t i = 0; i < nCell; ++i) { slow[i] = one/v[i]; }
+    }
+    else
+    {
+        for (int iz = 0; iz < nCellZ; ++iz)
+        {
+            for (int iy = 0; iy < nCellY; ++iy)
+            {
+                for (int ix = 0; ix < nCellX; ++ix)
+                {
+                    int64_t isrc = grid3(nCellZ, nCellY, iz, iy, ix);
+                    int64_t idst = grid3(nCellX, nCellY, ix, iy, iz);
+                    slow[idst] = one/v[isrc];
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/* src/solver3d.cpp:21-84. */
+int ekxo_initialize_travel_times3d(const ekxo_geometry3d *g,
+                                   const double *slow, double xs, double ys,
+                                   double zs, double *T)
+{
+    double off[3];
+    int cell[3], cellIndex;
+    if (ekxo_source3d(g, xs, ys, zs, off, cell, &cellIndex) != 0)
+    {
+        return -1;
+    }
+    size_t nGrid = (size_t) g->nx*g->ny*g->nz;
+    double sourceSlowness = slow[cellIndex];
+    for (size_t i = 0; i < nGrid; ++i) { T[i] = EKXO_HUGE; }
+    static const int corner[8][3] = {{0,0,0},{1,0,0},{0,1,0},{1,1,0},
+                                     {0,0,1},{1,0,1},{0,1,1},{1,1,1}};
+    for (int i = 0; i < 8; ++i)
+    {
+        int ix = cell[0] + corner[i][0];
+        int iy = cell[1] + corner[i][1];
+        int iz = cell[2] + corner[i][2];
+        T[grid3(g->nx, g->ny, ix, iy, iz)]
+            = ekxo_analytic_time3d(ix, iy, iz, g->dx, g->dy, g->dz,
+                                   off[0], off[1], off[2], sourceSlowness);
+    }
+    return 0;
+}
+
+static int uniform3d(const ekxo_geometry3d *g)
+{
+    const double eps100 = 2.220446049250313e-16*100;
+    return (fabs(g->dx - g->dy) < eps100 && fabs(g->dx - g->dz) < eps100);
+}
+
+/* One node of Solver3DSweep::updateFSM (include/private/solver3d.hpp:
+ * 186-243 uniform, :262-323 non-uniform). */
+static inline void update_node3d(const ekxo_geometry3d *g, const double *slow,
+                                 int uniform, int radius, int sweep,
+                                 const int *signs, const int *shifts,
+                                 const int *iSrc, const double *off,
+                                 double sSrc, int ix, int iy, int iz,
+                                 double *T)
+{
+    int iCell[7], it[8];
+    double s7[7], t7[7];
+    ekxo_slowness_indices3d(sweep, ix, iy, iz, g->nx - 1, g->ny - 1,
+                            g->nz - 1, iCell);
+    for (int k = 0; k < 7; ++k) { s7[k] = slow[iCell[k]]; }
+    ekxo_travel_time_indices3d(sweep, ix, iy, iz, g->nx, g->ny, g->nz, it);
+    double t0 = T[it[0]];
+    for (int k = 0; k < 7; ++k) { t7[k] = T[it[k + 1]]; }
+    double tUpd;
+    if (uniform)
+    {
+        tUpd = ekxo_fd3d_uniform(radius, EKXO_HUGE, g->dx, sSrc, ix, iy, iz,
+                                 signs, shifts, iSrc, off, s7, t7);
+    }
+    else
+    {
+        tUpd = ekxo_fd3d_nonuniform(radius, EKXO_HUGE, g->dx, g->dy, g->dz,
+                                    sSrc, ix, iy, iz, signs, shifts, iSrc,
+                                    off, s7, t7);
+    }
+    if (tUpd < t0) { T[it[0]] = tUpd; }
+}
+
+int ekxo_sweep3d(const ekxo_geometry3d *g, const double *slow,
+                 double xs, double ys, double zs, int radius, int sweep,
+                 int initialize, double *T)
+{
+    double off[3];
+    int iSrc[3], cellIndex;
+    if (sweep < 1 || sweep > 8) { return -1; }
+    if (ekxo_source3d(g, xs, ys, zs, off, iSrc, &cellIndex) != 0)
+    {
+        return -1;
+    }
+    /* src/solver3d.cpp:425-426: getSlowness(iSrcX, iSrcY, iSrcZ). */
+    double sSrc = slow[cellIndex];
+    int ix0, iy0, iz0, ix1, iy1, iz1, ixDir, iyDir, izDir;
+    if (initialize)
+    {
+        ekxo_loop_limits_init3d(sweep, g->nx, g->ny, g->nz,
+                                iSrc[0], iSrc[1], iSrc[2],
+                                &ix0, &iy0, &iz0, &ix1, &iy1, &iz1,
+                                &ixDir, &iyDir, &izDir);
+    }
+    else
+    {
+        ekxo_loop_limits3d(sweep, g->nx, g->ny, g->nz,
+                           &ix0, &iy0, &iz0, &ix1, &iy1, &iz1,
+                           &ixDir, &iyDir, &izDir);
+    }
+    int shifts[3], signs[3];
+    ekxo_sweep_signs3d(sweep, &shifts[0], &shifts[1], &shifts[2],
+                       &signs[0], &signs[1], &signs[2]);
+    int uniform = uniform3d(g);
+    for (int iz = iz0; iz != iz1; iz = iz + izDir)
+    {
+        for (int iy = iy0; iy != iy1; iy = iy + iyDir)
+        {
+            for (int ix = ix0; ix != ix1; ix = ix + ixDir)
+            {
+                update_node3d(g, slow, uniform, radius, sweep, signs, shifts,
+                              iSrc, off, sSrc, ix, iy, iz, T);
+            }
+        }
+    }
+    return 0;
+}
+
+int ekxo_solve3d(const ekxo_geometry3d *g, const double *slow,
+                 double xs, double ys, double zs, int nSweeps, int radius,
+                 int algorithm, int schedule, double *T)
+{
+    if (g->nx < 3 || g->ny < 3 || g->nz < 3) { return -1; }
+    if (ekxo_initialize_travel_times3d(g, slow, xs, ys, zs, T) != 0)
+    {
+        return -1;
+    }
+    if (schedule == EKXO_SCHEDULE_SHIPPED)
+    {
+        /* src/solver3d.cpp:451-463 (FSM) and :487-499 (level set). */
+        if (algorithm == EKXO_ALGORITHM_FAST_SWEEPING)
+        {
+            ekxo_sweep3d(g, slow, xs, ys, zs, radius, 1, 1, T);
+            ekxo_sweep3d(g, slow, xs, ys, zs, radius, 2, 1, T);
+        }
+        return 0;
+    }
+    for (int sweep = 1; sweep <= 8; ++sweep)
+    {
+        ekxo_sweep3d(g, slow, xs, ys, zs, radius, sweep, 1, T);
+    }
+    for (int k = 0; k < nSweeps; ++k)
+    {
+        for (int sweep = 1; sweep <= 8; ++sweep)
+        {
+            ekxo_sweep3d(g, slow, xs, ys, zs, radius, sweep, 0, T);
+        }
+    }
+    return 0;
+}
+
+/* Level-set order of one sweep: levels L = jx+jy+jz in the permuted frame,
+ * all nodes of a level independent (src/graph3d.cpp:119-221).  The update
+ * range (octant for init sweeps, upwind boundary excluded) is the same as
+ * the loop limits of the serial sweep. */
+static void sweep3d_level_set(const ekxo_geometry3d *g, const double *slow,
+                              int radius, int sweep, int initialize,
+                              const double *off, const int *iSrc,
+                              double sSrc, double *T)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    int ix0, iy0, iz0, ix1, iy1, iz1, ixDir, iyDir, izDir;
+    if (initialize)
+    {
+        ekxo_loop_limits_init3d(sweep, nx, ny, nz, iSrc[0], iSrc[1], iSrc[2],
+                                &ix0, &iy0, &iz0, &ix1, &iy1, &iz1,
+                                &ixDir, &iyDir, &izDir);
+    }
+    else
+    {
+        ekxo_loop_limits3d(sweep, nx, ny, nz, &ix0, &iy0, &iz0,
+                           &ix1, &iy1, &iz1, &ixDir, &iyDir, &izDir);
+    }
+    /* first permuted index that is updated in each axis */
+    int jx0 = ixDir > 0 ? ix0 : nx - 1 - ix0;
+    int jy0 = iyDir > 0 ? iy0 : ny - 1 - iy0;
+    int jz0 = izDir > 0 ? iz0 : nz - 1 - iz0;
+    int shifts[3], signs[3];
+    ekxo_sweep_signs3d(sweep, &shifts[0], &shifts[1], &shifts[2],
+                       &signs[0], &signs[1], &signs[2]);
+    int uniform = uniform3d(g);
+    int nLevels = nx + ny + nz - 2;
+    for (int level = jx0 + jy0 + jz0; level < nLevels; ++level)
+    {
+        int kLo = imax(jz0, level - (nx - 1) - (ny - 1));
+        int kHi = imin(nz - 1, level - jx0 - jy0);
+        if (kLo > kHi) { continue; }
+        int nPairs = (kHi - kLo + 1)*ny;
+        #pragma omp parallel for schedule(static)
+        for (int p = 0; p < nPairs; ++p)
+        {
+            int jz = kLo + p/ny;
+            int jy = p % ny;
+            int jx = level - jz - jy;
+            if (jy < jy0 || jx < jx0 || jx > nx - 1) { continue; }
+            int ix = ixDir > 0 ? jx : nx - 1 - jx;
+            int iy = iyDir > 0 ? jy : ny - 1 - jy;
+            int iz = izDir > 0 ? jz : nz - 1 - jz;
+            update_node3d(g, slow, uniform, radius, sweep, signs, shifts,
+                          iSrc, off, sSrc, ix, iy, iz, T);
+        }
+    }
+}
+
+int ekxo_solve3d_level_set(const ekxo_geometry3d *g, const double *slow,
+                           double xs, double ys, double zs, int nSweeps,
+                           int radius, int schedule, int nThreads, double *T)
+{
+    if (g->nx < 3 || g->ny < 3 || g->nz < 3) { return -1; }
+#ifdef _OPENMP
+    if (nThreads > 0) { omp_set_num_threads(nThreads); }
+#else
+    (void) nThreads;
+#endif
+    double off[3];
+    int iSrc[3], cellIndex;
+    if (ekxo_source3d(g, xs, ys, zs, off, iSrc, &cellIndex) != 0)
+    {
+        return -1;
+    }
+    if (ekxo_initialize_travel_times3d(g, slow, xs, ys, zs, T) != 0)
+    {
+        return -1;
+    }
+    double sSrc = slow[cellIndex];
+    if (schedule == EKXO_SCHEDULE_SHIPPED)
+    {
+        sweep3d_level_set(g, slow, radius, 1, 1, off, iSrc, sSrc, T);
+        sweep3d_level_set(g, slow, radius, 2, 1, off, iSrc, sSrc, T);
+        return 0;
+    }
+    for (int sweep = 1; sweep <= 8; ++sweep)
+    {
+        sweep3d_level_set(g, slow, radius, sweep, 1, off, iSrc, sSrc, T);
+    }
+    for (int k = 0; k < nSweeps; ++k)
+    {
+        for (int sweep = 1; sweep <= 8; ++sweep)
+        {
+            sweep3d_level_set(g, slow, radius, sweep, 0, off, iSrc, sSrc, T);
+        }
+    }
+    return 0;
+}
+
+/* src/analytic/homogeneous3d.cpp:14-70. */
+int ekxo_homogeneous3d(const ekxo_geometry3d *g, double xs, double ys,
+                       double zs, double vel, double *T)
+{
+    double xShifted = xs - g->x0, yShifted = ys - g->y0, zShifted = zs - g->z0;
+    double slowness = 1./vel;
+    for (int iz = 0; iz < g->nz; ++iz)
+    {
+        for (int iy = 0; iy < g->ny; ++iy)
+        {
+            for (int ix = 0; ix < g->nx; ++ix)
+            {
+                double delX = xShifted - ix*g->dx;
+                double delY = yShifted - iy*g->dy;
+                double delZ = zShifted - iz*g->dz;
+                T[grid3(g->nx, g->ny, ix, iy, iz)]
+                    = sqrt(delX*delX + delY*delY + delZ*delZ)*slowness;
+            }
+        }
+    }
+    return 0;
+}
+
+/* src/analytic/linearGradient3d.cpp:12-116 (gradient in z only). */
+int ekxo_linear_gradient3d(const ekxo_geometry3d *g, double xs, double ys,
+                           double zs, double vTop, double vBottom, double *T)
+{
+    double xShifted = xs - g->x0, yShifted = ys - g->y0, zShifted = zs - g->z0;
+    const double vGradInX = 0, vGradInY = 0;
+    double vGradInZ = (vBottom - vTop)/((g->nz - 1)*g->dz);
+    double absG2 = vGradInX*vGradInX + vGradInY*vGradInY + vGradInZ*vGradInZ;
+    double absG = sqrt(absG2);
+    double vel0 = vTop + vGradInZ*zShifted;
+    double slow0 = 1/vel0;
+    int varies = fabs(vTop - vBottom) > 1.e-14;
+    for (int iz = 0; iz < g->nz; ++iz)
+    {
+        for (int iy = 0; iy < g->ny; ++iy)
+        {
+            for (int ix = 0; ix < g->nx; ++ix)
+            {
+                double t;
+                if (varies)
+                {
+                    double delX = ix*g->dx - xShifted;
+                    double delY = iy*g->dy - yShifted;
+                    double delZ = iz*g->dz - zShifted;
+                    double dist2 = delX*delX + delY*delY + delZ*delZ;
+                    double vel = vel0 + vGradInX*delX + vGradInY*delY
+                               + vGradInZ*delZ;
+                    double arg = 1 + (slow0*absG2*dist2)/(2*vel);
+                    t = acosh(arg)/absG;
+                }
+                else
+                {
+                    double delX = xShifted - ix*g->dx;
+                    double delY = yShifted - iy*g->dy;
+                    double delZ = zShifted - iz*g->dz;
+                    t = sqrt(delX*delX + delY*delY + delZ*delZ)*slow0;
+                }
+                T[grid3(g->nx, g->ny, ix, iy, iz)] = t;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ======================================================================== */
+/*                                   2D                                     */
+/* ======================================================================== */
+
+static inline int grid2(int nx, int ix, int iz) { return iz*nx + ix; }
+
+/* solverUtilities2d.hpp:888-926.  Sweep1 +x+z, Sweep2 -x+z, Sweep3 +x-z,
+ * Sweep4 -x-z. */
+void ekxo_sweep_signs2d(int sweep, int *ixShift, int *izShift,
+                        int *signX, int *signZ)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negZ = (k >> 1) & 1;
+    *signX = negX ? -1 : 1;
+    *signZ = negZ ? -1 : 1;
+    *ixShift = negX ? 1 : -1;
+    *izShift = negZ ? 1 : -1;
+}
+
+/* solverUtilities2d.hpp:939-987. */
+void ekxo_loop_limits2d(int sweep, int nx, int nz, int *ix0, int *iz0,
+                        int *ix1, int *iz1, int *ixDir, int *izDir)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negZ = (k >> 1) & 1;
+    if (!negX) { *ix0 = 1;      *ix1 = nx; *ixDir = 1; }
+    else       { *ix0 = nx - 2; *ix1 = -1; *ixDir = -1; }
+    if (!negZ) { *iz0 = 1;      *iz1 = nz; *izDir = 1; }
+    else       { *iz0 = nz - 2; *iz1 = -1; *izDir = -1; }
+}
+
+/* solverUtilities2d.hpp:1004-1053. */
+void ekxo_loop_limits_init2d(int sweep, int iSrcX, int iSrcZ, int nx, int nz,
+                             int *ix0, int *iz0, int *ix1, int *iz1,
+                             int *ixDir, int *izDir)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negZ = (k >> 1) & 1;
+    if (!negX) { *ix0 = imax(iSrcX, 1);          *ix1 = nx; *ixDir = 1; }
+    else       { *ix0 = imin(iSrcX + 1, nx - 2); *ix1 = -1; *ixDir = -1; }
+    if (!negZ) { *iz0 = imax(iSrcZ, 1);          *iz1 = nz; *izDir = 1; }
+    else       { *iz0 = imin(iSrcZ + 1, nz - 2); *iz1 = -1; *izDir = -1; }
+}
+
+/* solverUtilities2d.hpp:1279-1370.  iCell3 = {c0, c1, c3}. */
+void ekxo_slowness_indices2d(int sweep, int ix, int iz, int nCellX,
+                             int nCellZ, int *iCell3)
+{
+    int k = sweep - 1;
+    int negX = k & 1, negZ = (k >> 1) & 1;
+    int c0x, c1x, c3x, c0z, c1z, c3z;
+    if (!negX)
+    {
+        c0x = imax(0, ix - 1);
+        c1x = imin(nCellX - 1, c0x + 1);
+        if (ix == 0) { c1x = 0; }
+        c3x = c0x;
+    }
+    else
+    {
+        c0x = ix;
+        c1x = imax(0, c0x - 1);
+        if (ix == nCellX) { c0x = ix - 1; c1x = ix - 1; }
+        c3x = c0x;
+    }
+    if (!negZ)
+    {
+        c0z = imax(0, iz - 1);
+        c3z = imin(nCellZ - 1, c0z + 1);
+        if (iz == 0) { c3z = 0; }
+        c1z = c0z;
+    }
+    else
+    {
+        c0z = iz;
+        c3z = imax(0, c0z - 1);
+        if (iz == nCellZ) { c0z = iz - 1; c3z = iz - 1; }
+        c1z = c0z;
+    }
+    iCell3[0] = grid2(nCellX, c0x, c0z);
+    iCell3[1] = grid2(nCellX, c1x, c1z);
+    iCell3[2] = grid2(nCellX, c3x, c3z);
+}
+
+/* solverUtilities2d.hpp:1384-1422 (no clamping; callers stay inside). */
+void ekxo_travel_time_indices2d(int sweep, int ix, int iz, int nx, int *it4)
+{
+    int k = sweep - 1;
+    int sx = (k & 1) ? 1 : -1;
+    int sz = ((k >> 1) & 1) ? 1 : -1;
+    it4[0] = grid2(nx, ix, iz);
+    it4[1] = grid2(nx, ix + sx, iz);
+    it4[2] = grid2(nx, ix + sx, iz + sz);
+    it4[3] = grid2(nx, ix, iz + sz);
+}
+
+/* solverUtilities2d.hpp:1230-1244. */
+void ekxo_level_start_stop2d(int nx, int nz, int level, int *i0, int *i1)
+{
+    *i0 = imax(0, level + 1 - nx);
+    *i1 = imin(nz, level + 1);
+}
+
+int ekxo_number_of_levels2d(int nx, int nz) { return nx + nz - 1; }
+
+/* solverUtilities2d.hpp:1065-1100. */
+void ekxo_sweep_level_index_to_grid2d(int sweep, int level, int indx, int nx,
+                                      int nz, int *ix, int *iz)
+{
+    int k = sweep - 1;
+    int jz = indx;
+    int jx = level - jz;
+    *ix = (k & 1) ? nx - 1 - jx : jx;
+    *iz = ((k >> 1) & 1) ? nz - 1 - jz : jz;
+}
+
+/* solverUtilities2d.hpp:1115-1147. */
+void ekxo_grid_to_level_index2d(int sweep, int ix, int iz, int nx, int nz,
+                                int *level, int *indx)
+{
+    int k = sweep - 1;
+    int jx = (k & 1) ? nx - 1 - ix : ix;
+    int jz = ((k >> 1) & 1) ? nz - 1 - iz : iz;
+    *indx = jz;
+    *level = jx + jz;
+}
+
+/* solverUtilities2d.hpp:1183-1221 (clamped neighbours). */
+void ekxo_level_travel_time_indices2d(int sweep, int level, int indx, int nx,
+                                      int nz, int *it4)
+{
+    int ix, iz;
+    ekxo_sweep_level_index_to_grid2d(sweep, level, indx, nx, nz, &ix, &iz);
+    int k = sweep - 1;
+    int ixS = (k & 1) ? imin(nx - 1, ix + 1) : imax(0, ix - 1);
+    int izS = ((k >> 1) & 1) ? imin(nz - 1, iz + 1) : imax(0, iz - 1);
+    it4[0] = grid2(nx, ix, iz);
+    it4[1] = grid2(nx, ixS, iz);
+    it4[2] = grid2(nx, ixS, izS);
+    it4[3] = grid2(nx, ix, izS);
+}
+
+/* solverUtilities2d.hpp:159-174 (distance by hypot). */
+double ekxo_analytic_time2d(int ix, int iz, double dx, double dz, double xs,
+                            double zs, double slow)
+{
+    double x = ix*dx;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaZ = z - zs;
+    double distance = hypot(deltaX, deltaZ);
+    return slow*distance;
+}
+
+/* solverUtilities2d.hpp:192-215. */
+static void analytic_time_deriv2d(int ix, int iz, double dx, double dz,
+                                  double xs, double zs, double slow,
+                                  double *t, double *dtdx, double *dtdz)
+{
+    double x = ix*dx;
+    double z = iz*dz;
+    double deltaX = x - xs;
+    double deltaZ = z - zs;
+    double distance = hypot(deltaX, deltaZ);
+    *t = slow*distance;
+    *dtdx = 0;
+    *dtdz = 0;
+    if (distance > 0)
+    {
+        *dtdx = (slow*deltaX)/distance;
+        *dtdz = (slow*deltaZ)/distance;
+    }
+}
+
+#define EKXO_SQRT1_2 0.70710678118654752440
+
+/* solverUtilities2d.hpp:345-472. */
+double ekxo_fd2d_uniform(int radius, double huge, double h, double sSrc,
+                         int ix, int iz, const int *signs, const int *shifts,
+                         const int *iSrc, const double *srcOffset,
+                         const double *s3, const double *t3v)
+{
+    const double s0 = s3[0], s1 = s3[1], s3s = s3[2];
+    const double t1 = t3v[0], t2 = t3v[1], t3 = t3v[2];
+    double tUpd = huge;
+    double t0x = t1 + h*fmin(s0, s3s);
+    double t0z = t3 + h*fmin(s0, s1);
+    double t0Update1 = fmin(t0x, t0z);
+    double hs0 = h*s0;
+    double dTx = t3 - t2;
+    double dTz = t1 - t2;
+    double sx = EKXO_SQRT1_2*s0;
+    double sz = EKXO_SQRT1_2*s0;
+    if (abs(iSrc[0] - ix) > radius || abs(iSrc[1] - iz) > radius)
+    {
+        double t0a = huge;
+        double deta = hs0*hs0 - dTz*dTz;
+        if (dTz >= 0 && deta >= 0 && dTz <= h*sz)
+        {
+            t0a = t1 + h*sqrt(deta);
+        }
+        double t0b = huge;
+        double detb = hs0*hs0 - dTx*dTx;
+        if (dTx >= 0 && detb >= 0 && dTx <= h*sx)
+        {
+            t0b = t3 + h*sqrt(detb);
+        }
+        double t0Update2 = fmin(t0a, t0b);
+        double dtCross = t1 - t3;
+        double detc = 2*(hs0*hs0) - dtCross*dtCross;
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && detc > 0 &&
+            t3 <= t1 + hs0 && t1 <= t3 + hs0)
+        {
+            t0Update3 = t2 + sqrt(detc);
+        }
+        tUpd = fmin(fmin(t0Update1, t0Update2), t0Update3);
+    }
+    else
+    {
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && t1 < t3 + hs0 && t3 < t1 + hs0)
+        {
+            double dt0dx, dt0dz, t0;
+            analytic_time_deriv2d(ix, iz, h, h, srcOffset[0], srcOffset[1],
+                                  sSrc, &t0, &dt0dx, &dt0dz);
+            double tau1 = t1 - ekxo_analytic_time2d(ix + shifts[0], iz, h, h,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            double tau2 = t2 - ekxo_analytic_time2d(ix + shifts[0],
+                                                    iz + shifts[1], h, h,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            double tau3 = t3 - ekxo_analytic_time2d(ix, iz + shifts[1], h, h,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            dt0dx = signs[0]*dt0dx;
+            dt0dz = signs[1]*dt0dz;
+            double tauX = tau1 - tau3 + tau2;
+            double tauZ = tau3 - tau1 + tau2;
+            const double one = 1, two = 2, four = 4;
+            double h2 = h*h;
+            double a = one/h2;
+            double b = two/h*(dt0dx + dt0dz) - two/h2*tau2;
+            double c = one/(two*h2)*(tauX*tauX + tauZ*tauZ)
+                     - two/h*(dt0dx*tauX + dt0dz*tauZ)
+                     + two*(sSrc*sSrc - s0*s0);
+            double det = b*b - four*a*c;
+            if (det >= 0)
+            {
+                double tau = (-b + sqrt(det))/(two*a);
+                t0Update3 = t0 + tau;
+            }
+            if (t0Update3 < t1 || t0Update3 < t3) { t0Update3 = huge; }
+        }
+        tUpd = fmin(t0Update1, t0Update3);
+    }
+    return tUpd;
+}
+
+/* solverUtilities2d.hpp:474-614; derived constants follow
+ * include/private/solver2d.hpp:41-46. */
+double ekxo_fd2d_nonuniform(int radius, double huge, double dx, double dz,
+                            double sSrc, int ix, int iz, const int *signs,
+                            const int *shifts, const int *iSrc,
+                            const double *srcOffset, const double *s3,
+                            const double *t3v)
+{
+    double dx_dz = dx/dz;
+    double dz_dx = dz/dx;
+    double cosTheta = dx/hypot(dx, dz);
+    double sinTheta = dz/hypot(dx, dz);
+    const double s0 = s3[0], s1 = s3[1], s3s = s3[2];
+    const double t1 = t3v[0], t2 = t3v[1], t3 = t3v[2];
+    double tUpd = huge;
+    double t0x = t1 + dx*fmin(s0, s3s);
+    double t0z = t3 + dz*fmin(s0, s1);
+    double t0Update1 = fmin(t0x, t0z);
+    double dTx = t3 - t2;
+    double dTz = t1 - t2;
+    double sx = cosTheta*s0;
+    double sz = sinTheta*s0;
+    double hxs0 = dx*s0;
+    double hzs0 = dz*s0;
+    if (abs(iSrc[0] - ix) > radius || abs(iSrc[1] - iz) > radius)
+    {
+        double t0a = huge;
+        double deta = hzs0*hzs0 - dTz*dTz;
+        if (dTz >= 0 && deta >= 0 && dTz <= dz*sz)
+        {
+            t0a = t1 + dx_dz*sqrt(deta);
+        }
+        double t0b = huge;
+        double detb = hxs0*hxs0 - dTx*dTx;
+        if (dTx >= 0 && detb >= 0 && dTx <= dx*sx)
+        {
+            t0b = t3 + dz_dx*sqrt(detb);
+        }
+        double t0Update2 = fmin(t0a, t0b);
+        double dxdz_dx2_p_dz2 = cosTheta*sinTheta;
+        double tildeTx = t1 - t3 + t2;
+        double tildeTz = t3 - t1 + t2;
+        double dtCross = t1 - t3;
+        double detc = hxs0*hxs0 + hzs0*hzs0 - dtCross*dtCross;
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && detc > 0 &&
+            t1 < t3 + hzs0 && t3 < t1 + hxs0)
+        {
+            t0Update3 = tildeTx*dz_dx + tildeTz*dx_dz + 2*sqrt(detc);
+            t0Update3 = dxdz_dx2_p_dz2*t0Update3;
+        }
+        if (t0Update3 < 0) { t0Update3 = huge; }
+        tUpd = fmin(fmin(t0Update1, t0Update2), t0Update3);
+    }
+    else
+    {
+        double t0Update3 = huge;
+        if (dTx >= 0 && dTz >= 0 && t1 < t3 + hzs0 && t3 < t1 + hxs0)
+        {
+            double dt0dx, dt0dz, t0;
+            analytic_time_deriv2d(ix, iz, dx, dz, srcOffset[0], srcOffset[1],
+                                  sSrc, &t0, &dt0dx, &dt0dz);
+            double tau1 = t1 - ekxo_analytic_time2d(ix + shifts[0], iz, dx, dz,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            double tau2 = t2 - ekxo_analytic_time2d(ix + shifts[0],
+                                                    iz + shifts[1], dx, dz,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            double tau3 = t3 - ekxo_analytic_time2d(ix, iz + shifts[1], dx, dz,
+                                                    srcOffset[0],
+                                                    srcOffset[1], sSrc);
+            dt0dx = signs[0]*dt0dx;
+            dt0dz = signs[1]*dt0dz;
+            double tauX = tau1 - tau3 + tau2;
+            double tauZ = tau3 - tau1 + tau2;
+            const double one = 1, two = 2, four = 4;
+            double hx2inv = one/(dx*dx);
+            double hz2inv = one/(dz*dz);
+            double dxinv = one/dx;
+            double dzinv = one/dz;
+            double a = hx2inv + hz2inv;
+            double b = four*(dxinv*dt0dx + dzinv*dt0dz)
+                     - two*(tauX*hx2inv + tauZ*hz2inv);
+            double c = (tauX*tauX)*hx2inv + (tauZ*tauZ)*hz2inv
+                     - four*((dt0dx*tauX)*dxinv + (dt0dz*tauZ)*dzinv)
+                     + four*(sSrc*sSrc - s0*s0);
+            double det = b*b - four*a*c;
+            if (det >= 0)
+            {
+                double tau = (-b + sqrt(det))/(two*a);
+                t0Update3 = t0 + tau;
+            }
+            if (t0Update3 < t1 || t0Update3 < t3) { t0Update3 = huge; }
+        }
+        tUpd = fmin(t0Update1, t0Update3);
+    }
+    return tUpd;
+}
+
+/* src/source2d.cpp (same arithmetic as the 3D source, two axes). */
+int ekxo_source2d(const ekxo_geometry2d *g, double x, double z,
+                  double *offset2, int *cell2, int *cellIndex)
+{
+    int nCellX = g->nx - 1, nCellZ = g->nz - 1;
+    double x1 = g->x0 + (double) (g->nx - 1)*g->dx;
+    double z1 = g->z0 + (double) (g->nz - 1)*g->dz;
+    if (x < g->x0 || x > x1) { return -1; }
+    if (z < g->z0 || z > z1) { return -1; }
+    offset2[0] = x - g->x0;
+    offset2[1] = z - g->z0;
+    cell2[0] = imin(nCellX - 1, (int) (offset2[0]/g->dx));
+    cell2[1] = imin(nCellZ - 1, (int) (offset2[1]/g->dz));
+    *cellIndex = grid2(nCellX, cell2[0], cell2[1]);
+    return 0;
+}
+
+/* src/model2d.cpp:46-127. */
+int ekxo_nodal_to_cell2d(int nx, int nz, const double *v, int ordering,
+                         double *slow)
+{
+    size_t nGrid = (size_t) nx*nz;
+    for (size_t i = 0; i < nGrid; ++i) { if (!(v[i] > 0)) { return -1; } }
+    size_t nCellX = (size_t) nx - 1, nCellZ = (size_t) nz - 1;
+    const double four = 4;
+    for (size_t iz = 0; iz < nCellZ; ++iz)
+    {
+        for (size_t ix = 0; ix < nCellX; ++ix)
+        {
+            size_t idst = iz*nCellX + ix;
+            size_t idx1, idx2, idx3, idx4;
+            if (ordering == 0)
+            {
+                idx1 = iz*(nCellX + 1) + ix;
+                idx2 = idx1 + 1;
+                idx3 = idx1 + (nCellX + 1);
+                idx4 = idx3 + 1;
+            }
+            else
+            {
+                idx1 = ix*(nCellZ + 1) + iz;
+                idx2 = idx1 + 1;
+                idx3 = idx1 + (nCellZ + 1);
+                idx4 = idx3 + 1;
+            }
+            double sum1 = v[idx1] + v[idx2];
+            double sum2 = v[idx3] + v[idx4];
+            slow[idst] = four/(sum1 + sum2);
+        }
+    }
+    return 0;
+}
+
+static int uniform2d(const ekxo_geometry2d *g)
+{
+    const double eps100 = 2.220446049250313e-16*100;
+    return fabs(g->dx - g->dz) < eps100;
+}
+
+/* include/private/solver2d.hpp:203-327 (Solver2DSweep::updateFSM). */
+int ekxo_sweep2d(const ekxo_geometry2d *g, const double *slow, double xs,
+                 double zs, int radius, int sweep, int initialize, double *T)
+{
+    double off[2];
+    int iSrc[2], cellIndex;
+    if (sweep < 1 || sweep > 4) { return -1; }
+    if (ekxo_source2d(g, xs, zs, off, iSrc, &cellIndex) != 0) { return -1; }
+    double sSrc = slow[cellIndex];
+    int ix0, iz0, ix1, iz1, ixDir, izDir;
+    if (initialize)
+    {
+        ekxo_loop_limits_init2d(sweep, iSrc[0], iSrc[1], g->nx, g->nz,
+                                &ix0, &iz0, &ix1, &iz1, &ixDir, &izDir);
+    }
+    else
+    {
+        ekxo_loop_limits2d(sweep, g->nx, g->nz, &ix0, &iz0, &ix1, &iz1,
+                           &ixDir, &izDir);
+    }
+    int shifts[2], signs[2];
+    ekxo_sweep_signs2d(sweep, &shifts[0], &shifts[1], &signs[0], &signs[1]);
+    int uniform = uniform2d(g);
+    for (int iz = iz0; iz != iz1; iz = iz + izDir)
+    {
+        for (int ix = ix0; ix != ix1; ix = ix + ixDir)
+        {
+            int iCell[3], it[4];
+            double s3[3], t3[3];
+            ekxo_slowness_indices2d(sweep, ix, iz, g->nx - 1, g->nz - 1,
+                                    iCell);
+            s3[0] = slow[iCell[0]];
+            s3[1] = slow[iCell[1]];
+            s3[2] = slow[iCell[2]];
+            ekxo_travel_time_indices2d(sweep, ix, iz, g->nx, it);
+            double t0 = T[it[0]];
+            t3[0] = T[it[1]];
+            t3[1] = T[it[2]];
+            t3[2] = T[it[3]];
+            double tUpd;
+            if (uniform)
+            {
+                tUpd = ekxo_fd2d_uniform(radius, EKXO_HUGE, g->dx, sSrc,
+                                         ix, iz, signs, shifts, iSrc, off,
+                                         s3, t3);
+            }
+            else
+            {
+                tUpd = ekxo_fd2d_nonuniform(radius, EKXO_HUGE, g->dx, g->dz,
+                                            sSrc, ix, iz, signs, shifts,
+                                            iSrc, off, s3, t3);
+            }
+            if (tUpd < t0) { T[it[0]] = tUpd; }
+        }
+    }
+    return 0;
+}
+
+/* src/solver2d.cpp:61-105 and :462-565 (fast-sweeping branch). */
+int ekxo_solve2d(const ekxo_geometry2d *g, const double *slow, double xs,
+                 double zs, int nSweeps, int radius, double *T)
+{
+    if (g->nx < 3 || g->nz < 3) { return -1; }
+    double off[2];
+    int cell[2], cellIndex;
+    if (ekxo_source2d(g, xs, zs, off, cell, &cellIndex) != 0) { return -1; }
+    double sourceSlowness = slow[cellIndex];
+    size_t nGrid = (size_t) g->nx*g->nz;
+    for (size_t i = 0; i < nGrid; ++i) { T[i] = EKXO_HUGE; }
+    static const int corner[4][2] = {{0,0},{1,0},{0,1},{1,1}};
+    for (int i = 0; i < 4; ++i)
+    {
+        int ix = cell[0] + corner[i][0];
+        int iz = cell[1] + corner[i][1];
+        T[grid2(g->nx, ix, iz)]
+            = ekxo_analytic_time2d(ix, iz, g->dx, g->dz, off[0], off[1],
+                                   sourceSlowness);
+    }
+    for (int sweep = 1; sweep <= 4; ++sweep)
+    {
+        ekxo_sweep2d(g, slow, xs, zs, radius, sweep, 1, T);
+    }
+    for (int k = 0; k < nSweeps; ++k)
+    {
+        for (int sweep = 1; sweep <= 4; ++sweep)
+        {
+            ekxo_sweep2d(g, slow, xs, zs, radius, sweep, 0, T);
+        }
+    }
+    return 0;
+}
+
+/* src/analytic/homogeneous2d.cpp:14-60. */
+int ekxo_homogeneous2d(const ekxo_geometry2d *g, double xs, double zs,
+                       double vel, double *T)
+{
+    double xShifted = xs - g->x0, zShifted = zs - g->z0;
+    double slowness = 1./vel;
+    for (int iz = 0; iz < g->nz; ++iz)
+    {
+        for (int ix = 0; ix < g->nx; ++ix)
+        {
+            double delX = xShifted - ix*g->dx;
+            double delZ = zShifted - iz*g->dz;
+            T[grid2(g->nx, ix, iz)] = hypot(delX, delZ)*slowness;
+        }
+    }
+    return 0;
+}
+
+/* src/analytic/linearGradient2d.cpp:12-110. */
+int ekxo_linear_gradient2d(const ekxo_geometry2d *g, double xs, double zs,
+                           double vTop, double vBottom, double *T)
+{
+    double xShifted = xs - g->x0, zShifted = zs - g->z0;
+    const double vGradInX = 0;
+    double vGradInZ = (vBottom - vTop)/((g->nz - 1)*g->dz);
+    double absG2 = vGradInX*vGradInX + vGradInZ*vGradInZ;
+    double absG = sqrt(absG2);
+    double vel0 = vTop + vGradInZ*zShifted;
+    double slow0 = 1/vel0;
+    int varies = fabs(vTop - vBottom) > 1.e-14;
+    for (int iz = 0; iz < g->nz; ++iz)
+    {
+        for (int ix = 0; ix < g->nx; ++ix)
+        {
+            double t;
+            if (varies)
+            {
+                double delX = ix*g->dx - xShifted;
+                double delZ = iz*g->dz - zShifted;
+                double dist2 = delX*delX + delZ*delZ;
+                double vel = vel0 + vGradInX*delX + vGradInZ*delZ;
+                double arg = 1 + (slow0*absG2*dist2)/(2*vel);
+                t = acosh(arg)/absG;
+            }
+            else
+            {
+                double delX = xShifted - ix*g->dx;
+                double delZ = zShifted - iz*g->dz;
+                t = hypot(delX, delZ)*slow0;
+            }
+            T[grid2(g->nx, ix, iz)] = t;
+        }
+    }
+    return 0;
+}
+
+/* src/solver2d.cpp:106-167 with src/ray/bilinear.hpp. */
+double ekxo_station_time2d(const ekxo_geometry2d *g, const double *slow,
+                           const double *T, double xs, double zs, double xr,
+                           double zr)
+{
+    double soff[2], roff[2];
+    int scell[2], rcell[2], sIndex, rIndex;
+    if (ekxo_source2d(g, xs, zs, soff, scell, &sIndex) != 0) { return -1; }
+    if (ekxo_source2d(g, xr, zr, roff, rcell, &rIndex) != 0) { return -1; }
+    double sourceSlowness = slow[sIndex];
+    double dx = g->dx, dz = g->dz;
+    double dxi = 1./dx;
+    double dzi = 1./dz;
+    double xi = roff[0], zi = roff[1];
+    if (rIndex == sIndex)
+    {
+        return hypot(xi - soff[0], zi - soff[1])*sourceSlowness;
+    }
+    int i00 = grid2(g->nx, rcell[0], rcell[1]);
+    int i10 = i00 + 1;
+    int i01 = i00 + g->nx;
+    int i11 = i01 + 1;
+    double x0 = rcell[0]*dx;
+    double x1 = (rcell[0] + 1)*dx;
+    double z0 = rcell[1]*dz;
+    double z1 = (rcell[1] + 1)*dz;
+    double f00 = T[i00], f01 = T[i01], f10 = T[i10], f11 = T[i11];
+    double x2mx = x1 - xi;
+    double xmx1 = xi - x0;
+    double fxz1 = dxi*(x2mx*f00 + xmx1*f10);
+    double fxz2 = dxi*(x2mx*f01 + xmx1*f11);
+    double z2mz = z1 - zi;
+    double zmz1 = zi - z0;
+    return dzi*(z2mz*fxz1 + zmz1*fxz2);
+}

@@ -1,0 +1,192 @@
+/*
+ * ekx_oracle.h -- CPU restatement of the reference eikonal arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library (eikonalxx_amd) never links or calls it.
+ *
+ * Every function restates one reference function line by line, in double
+ * precision, compiled with -ffp-contract=off so that the operation order of
+ * the reference source is kept (no fused multiply-add).  Citations are to
+ * files of uofuseismo/eikonalxx.
+ *
+ * Pinning status (what checks each function besides its own transcription):
+ *   index helpers (signs, loop limits, slowness/travel-time indices, permute,
+ *   boundary test, Graph3D level lists, 2D level helpers)
+ *       -> reference test closed forms and golden vectors
+ *          (testing/solver3d.cpp, testing/graph.cpp, testing/solver2d.cpp).
+ *   source location -> cell     -> testing/source.cpp closed forms.
+ *   nodal -> cell slowness      -> testing/model.cpp closed forms.
+ *   analytic fields             -> closed forms (testing/analytic.cpp).
+ *   3D/2D local solvers         -> PARTIAL: plane-wave exactness of the 1D,
+ *       2D and 8-point operators, the factored operators returning T0 for
+ *       T0 neighbours in homogeneous media, monotonicity/causality
+ *       invariants, and convergence to the analytic fields.  Bitwise parity
+ *       with the reference binary itself is unpinned: the reference needs
+ *       SYCL and TBB headers that this image lacks.
+ */
+#ifndef EKX_ORACLE_H
+#define EKX_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EKXO_HUGE 1.e10
+
+typedef struct
+{
+    int nx, ny, nz;
+    double dx, dy, dz;
+    double x0, y0, z0;
+} ekxo_geometry3d;
+
+typedef struct
+{
+    int nx, nz;
+    double dx, dz;
+    double x0, z0;
+} ekxo_geometry2d;
+
+/* Schedules of Solver3D::solve.  SHIPPED is what src/solver3d.cpp:451-499
+ * does today (FSM: init sweeps 1 and 2 then return; LSM: nothing after the
+ * source initialisation).  COMPLETED mirrors Solver2D::solve
+ * (src/solver2d.cpp:514-539): 8 init sweeps then nSweeps x 8 refinement
+ * sweeps. */
+enum { EKXO_SCHEDULE_COMPLETED = 0, EKXO_SCHEDULE_SHIPPED = 1 };
+enum { EKXO_ALGORITHM_LEVEL_SET = 0, EKXO_ALGORITHM_FAST_SWEEPING = 1 };
+
+/* ---- 3D index helpers (include/private/solverUtilities3d.hpp) ---------- */
+void ekxo_sweep_signs3d(int sweep, int *ixShift, int *iyShift, int *izShift,
+                        int *signX, int *signY, int *signZ);
+void ekxo_loop_limits3d(int sweep, int nx, int ny, int nz,
+                        int *ix0, int *iy0, int *iz0,
+                        int *ix1, int *iy1, int *iz1,
+                        int *ixDir, int *iyDir, int *izDir);
+void ekxo_loop_limits_init3d(int sweep, int nx, int ny, int nz,
+                             int iSrcX, int iSrcY, int iSrcZ,
+                             int *ix0, int *iy0, int *iz0,
+                             int *ix1, int *iy1, int *iz1,
+                             int *ixDir, int *iyDir, int *izDir);
+void ekxo_permute3d(int sweep, int ix, int iy, int iz, int nx, int ny, int nz,
+                    int *jx, int *jy, int *jz);
+void ekxo_slowness_indices3d(int sweep, int ix, int iy, int iz,
+                             int nCellX, int nCellY, int nCellZ, int *iCell7);
+void ekxo_travel_time_indices3d(int sweep, int ix, int iy, int iz,
+                                int nx, int ny, int nz, int *it8);
+int ekxo_is_sweep_boundary3d(int sweep, int ix, int iy, int iz,
+                             int nx, int ny, int nz);
+int ekxo_graph3d_number_of_levels(int nx, int ny, int nz);
+/* levelStart[nLevels+1], nodeToIndex[nx*ny*nz] */
+int ekxo_graph3d(int sweep, int nx, int ny, int nz,
+                 int *levelStart, int *nodeToIndex);
+
+/* ---- 3D analytic and local solvers ------------------------------------- */
+double ekxo_analytic_time3d(int ix, int iy, int iz,
+                            double dx, double dy, double dz,
+                            double xs, double ys, double zs, double slow);
+void ekxo_analytic_time_deriv3d(int ix, int iy, int iz,
+                                double dx, double dy, double dz,
+                                double xs, double ys, double zs, double slow,
+                                double *t, double *dtdx, double *dtdy,
+                                double *dtdz);
+/* s7 = {s0,s1,s2,s3,s4,s5,s7}; t7 = {t1..t7}; ints: signs[3], shifts[3],
+ * iSrc[3]; srcOffset[3] */
+double ekxo_fd3d_uniform(int radius, double huge, double h, double sSrc,
+                         int ix, int iy, int iz, const int *signs,
+                         const int *shifts, const int *iSrc,
+                         const double *srcOffset, const double *s7,
+                         const double *t7);
+double ekxo_fd3d_nonuniform(int radius, double huge,
+                            double dx, double dy, double dz, double sSrc,
+                            int ix, int iy, int iz, const int *signs,
+                            const int *shifts, const int *iSrc,
+                            const double *srcOffset, const double *s7,
+                            const double *t7);
+
+/* ---- 3D source / model / solver ---------------------------------------- */
+/* returns 0 on success, -1 if the location is outside the model */
+int ekxo_source3d(const ekxo_geometry3d *g, double x, double y, double z,
+                  double *offset3, int *cell3, int *cellIndex);
+/* ordering 0: natural (x fastest), 1: XYZ (z fastest) */
+int ekxo_nodal_to_cell3d(int nx, int ny, int nz, const double *v,
+                         int ordering, double *slow);
+int ekxo_cell_velocity_to_slowness3d(int nx, int ny, int nz, const double *v,
+                                     int ordering, double *slow);
+/* Fills T with HUGE and sets the 8 source-cell corners. */
+int ekxo_initialize_travel_times3d(const ekxo_geometry3d *g,
+                                   const double *slow, double xs, double ys,
+                                   double zs, double *T);
+/* One serial FSM sweep (Solver3DSweep::updateFSM). sweep in 1..8. */
+int ekxo_sweep3d(const ekxo_geometry3d *g, const double *slow,
+                 double xs, double ys, double zs, int radius, int sweep,
+                 int initialize, double *T);
+/* Full solve with the serial FSM loop order. */
+int ekxo_solve3d(const ekxo_geometry3d *g, const double *slow,
+                 double xs, double ys, double zs, int nSweeps, int radius,
+                 int algorithm, int schedule, double *T);
+/* Same schedule with hyperplane-parallel (level-set) order; OpenMP over the
+ * nodes of a level.  nThreads <= 0 uses the OpenMP default. */
+int ekxo_solve3d_level_set(const ekxo_geometry3d *g, const double *slow,
+                           double xs, double ys, double zs, int nSweeps,
+                           int radius, int schedule, int nThreads, double *T);
+
+/* ---- 3D analytic fields (src/analytic/) -------------------------------- */
+int ekxo_homogeneous3d(const ekxo_geometry3d *g, double xs, double ys,
+                       double zs, double vel, double *T);
+int ekxo_linear_gradient3d(const ekxo_geometry3d *g, double xs, double ys,
+                           double zs, double vTop, double vBottom, double *T);
+
+/* ---- 2D (include/private/solverUtilities2d.hpp, src/solver2d.cpp) ------ */
+void ekxo_sweep_signs2d(int sweep, int *ixShift, int *izShift,
+                        int *signX, int *signZ);
+void ekxo_loop_limits2d(int sweep, int nx, int nz, int *ix0, int *iz0,
+                        int *ix1, int *iz1, int *ixDir, int *izDir);
+void ekxo_loop_limits_init2d(int sweep, int iSrcX, int iSrcZ, int nx, int nz,
+                             int *ix0, int *iz0, int *ix1, int *iz1,
+                             int *ixDir, int *izDir);
+void ekxo_slowness_indices2d(int sweep, int ix, int iz, int nCellX,
+                             int nCellZ, int *iCell3);
+void ekxo_travel_time_indices2d(int sweep, int ix, int iz, int nx, int *it4);
+void ekxo_level_start_stop2d(int nx, int nz, int level, int *i0, int *i1);
+int ekxo_number_of_levels2d(int nx, int nz);
+void ekxo_sweep_level_index_to_grid2d(int sweep, int level, int indx, int nx,
+                                      int nz, int *ix, int *iz);
+void ekxo_grid_to_level_index2d(int sweep, int ix, int iz, int nx, int nz,
+                                int *level, int *indx);
+void ekxo_level_travel_time_indices2d(int sweep, int level, int indx, int nx,
+                                      int nz, int *it4);
+double ekxo_analytic_time2d(int ix, int iz, double dx, double dz, double xs,
+                            double zs, double slow);
+/* s3 = {s0,s1,s3}; t3 = {t1,t2,t3}; signs[2], shifts[2], iSrc[2],
+ * srcOffset[2] */
+double ekxo_fd2d_uniform(int radius, double huge, double h, double sSrc,
+                         int ix, int iz, const int *signs, const int *shifts,
+                         const int *iSrc, const double *srcOffset,
+                         const double *s3, const double *t3);
+double ekxo_fd2d_nonuniform(int radius, double huge, double dx, double dz,
+                            double sSrc, int ix, int iz, const int *signs,
+                            const int *shifts, const int *iSrc,
+                            const double *srcOffset, const double *s3,
+                            const double *t3);
+int ekxo_source2d(const ekxo_geometry2d *g, double x, double z,
+                  double *offset2, int *cell2, int *cellIndex);
+int ekxo_nodal_to_cell2d(int nx, int nz, const double *v, int ordering,
+                         double *slow);
+int ekxo_sweep2d(const ekxo_geometry2d *g, const double *slow, double xs,
+                 double zs, int radius, int sweep, int initialize, double *T);
+int ekxo_solve2d(const ekxo_geometry2d *g, const double *slow, double xs,
+                 double zs, int nSweeps, int radius, double *T);
+int ekxo_homogeneous2d(const ekxo_geometry2d *g, double xs, double zs,
+                       double vel, double *T);
+int ekxo_linear_gradient2d(const ekxo_geometry2d *g, double xs, double zs,
+                           double vTop, double vBottom, double *T);
+/* Bilinear travel time at a point (src/solver2d.cpp:106-167). */
+double ekxo_station_time2d(const ekxo_geometry2d *g, const double *slow,
+                           const double *T, double xs, double zs, double xr,
+                           double zr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
