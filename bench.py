@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X fast-sweeping eikonal solver.
+
+Metric: grid-point updates/s (one local-solver evaluation at one node in
+one sweep) and the fraction of the HBM roofline, for the batched
+FP32-with-FP64-polish solve against the plain FP64 solve, at <= 1e-5
+relative travel-time error.
+
+A step is one complete batched solve of the workload (all sources, the
+whole sweep schedule) with the model and every buffer resident in HBM.
+The default workload is the station-reciprocal travel-time tables: 256
+sources on one shared 385 x 385 x 193 model, h = 50 m, 4 iterations.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"fp32": 12.0, "fp64": 24.0}
+
+
+def log(*args):
+    print("[bench]", *args, file=sys.stderr, flush=True)
+
+
+def rel_error(t, t64, s_src, h):
+    """|dT| / max(T64, s_src h), the error measure of the contract."""
+    den = np.maximum(t64, s_src * h)
+    r = np.abs(t - t64) / den
+    return float(r.max()), float(np.sqrt(np.mean(r * r)))
+
+
+def build_tables_workload(args):
+    from eikonalxx_amd import workloads
+    w = workloads.config5_tables(nx=args.nx, ny=args.ny, nz=args.nz,
+                                 h=args.h, n_sources=args.sources)
+    slow = workloads.cell_slowness(w["velocity"], w["nx"], w["ny"], w["nz"])
+    return w, slow
+
+
+def cpu_baseline(w, slow, source, threads, iterations):
+    """The oracle's hyperplane-parallel FP64 solve (OpenMP) of one source on
+    the full grid: a bounded sample of the same workload."""
+    import oracle
+    g = oracle.geometry3d(w["nx"], w["ny"], w["nz"], w["h"])
+    nx, ny, nz = w["nx"], w["ny"], w["nz"]
+    t0 = time.perf_counter()
+    oracle.solve3d_level_set(g, slow, source, iterations, w["radius"],
+                             threads=threads)
+    dt = time.perf_counter() - t0
+    # updates of that schedule: 8 init octant sweeps + iterations x 8
+    _, cell, _ = oracle.source3d(g, *source)
+    upd = 0
+    for sweep in range(8):
+        negx, negy, negz = sweep & 1, (sweep >> 1) & 1, (sweep >> 2) & 1
+        j0 = [max(n - 2 - c, 1) if neg else max(1, c)
+              for n, c, neg in ((nx, cell[0], negx), (ny, cell[1], negy),
+                                (nz, cell[2], negz))]
+        upd += (nx - j0[0]) * (ny - j0[1]) * (nz - j0[2])
+    upd += iterations * 8 * (nx - 1) * (ny - 1) * (nz - 1)
+    return upd / dt, dt, upd
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sources", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=385)
+    ap.add_argument("--ny", type=int, default=385)
+    ap.add_argument("--nz", type=int, default=193)
+    ap.add_argument("--h", type=float, default=50.0)
+    ap.add_argument("--check-sources", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-iterations", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one "
+              "GPU and the solver has no multi-GPU path (sources would be "
+              "sharded across GPUs with no collective); run with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import eikonalxx_amd as ekx
+    if ekx.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    t0 = time.time()
+    w, slow = build_tables_workload(args)
+    nx, ny, nz, h = w["nx"], w["ny"], w["nz"], w["h"]
+    geom = ekx.Geometry3D(nx, ny, nz, h)
+    log("model %dx%dx%d h=%g, %d sources, built in %.1fs"
+        % (nx, ny, nz, h, len(w["sources"]), time.time() - t0))
+
+    # ---- batched FP32 sweeps with the final iteration in FP64 ----------
+    opts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
+                             factored_radius=w["radius"],
+                             precision=ekx.Precision.FP32_POLISH)
+    batch = ekx.Batch3D()
+    batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
+    log("batch initialized (%.1f GB of fields)"
+        % (len(w["sources"]) * geom.n_grid * 12 / 1e9))
+    for i in range(args.warmup):
+        batch.solve()
+        log("warmup %d: %.1f ms" % (i, batch.get_stats().last_solve_ms))
+    ekx.synchronize()
+    wall0 = time.perf_counter()
+    dev_ms, f32_ms, f64_ms = [], [], []
+    for i in range(args.steps):
+        batch.solve()
+        st = batch.get_stats()
+        dev_ms.append(st.last_solve_ms)
+        f32_ms.append(st.sweep_ms_fp32)
+        f64_ms.append(st.sweep_ms_fp64)
+        log("step %d: %.1f ms (fp32 sweeps %.1f ms, fp64 polish %.1f ms)"
+            % (i, st.last_solve_ms, st.sweep_ms_fp32, st.sweep_ms_fp64))
+    ekx.synchronize()
+    wall = time.perf_counter() - wall0
+    st = batch.get_stats()
+    ms_per_step = 1000.0 * wall / args.steps
+    updates = st.updates
+    updates64 = st.updates_fp64
+    updates32 = updates - updates64
+    value = updates / (ms_per_step / 1000.0)
+    launches32 = st.kernel_launches - st.kernel_launches_fp64
+    f32 = float(np.mean(f32_ms))
+    achieved = updates32 * BYTES_PER_UPDATE["fp32"] / (f32 / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "sweep_columns3d<float>",
+                "bytes_per_update": BYTES_PER_UPDATE["fp32"],
+                "updates_per_step": updates32,
+                "launches_per_step": launches32,
+                "avg_launch_us": round(1000.0 * f32 / launches32, 2)}
+    log("value %.3e updates/s, fp32 column kernel %.1f GB/s"
+        % (value, achieved))
+
+    # ---- accuracy against FP64 on sampled sources -------------------------
+    rng = np.random.default_rng(99)
+    picks = sorted(rng.choice(len(w["sources"]), size=min(
+        args.check_sources, len(w["sources"])), replace=False).tolist())
+    ref_opts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
+                                 factored_radius=w["radius"],
+                                 precision=ekx.Precision.FP64)
+    ref = ekx.Batch3D()
+    ref.initialize(ref_opts, [(geom, slow, w["sources"][i]) for i in picks])
+    ref.solve()
+    max_err, rms_acc = 0.0, 0.0
+    for k, i in enumerate(picks):
+        t64 = ref.get_travel_time_field(k)
+        t = batch.get_travel_time_field(i)
+        x, y, z = w["sources"][i]
+        ix = min(nx - 2, int(x / h))
+        iy = min(ny - 2, int(y / h))
+        iz = min(nz - 2, int(z / h))
+        s_src = slow[(iz * (ny - 1) + iy) * (nx - 1) + ix]
+        mx, rms = rel_error(t, t64, s_src, h)
+        max_err = max(max_err, mx)
+        rms_acc += rms * rms
+    rms_err = float(np.sqrt(rms_acc / len(picks)))
+    del ref
+    log("accuracy vs FP64 on %d sources: max %.2e rms %.2e"
+        % (len(picks), max_err, rms_err))
+
+    # ---- the plain FP64 solve (one source) ---------------------------------
+    plain = ekx.Solver3D()
+    plain.initialize(ref_opts, geom)
+    plain.set_slowness(slow)
+    plain.set_source(w["sources"][picks[0]])
+    plain.solve()
+    plain_ms = []
+    for _ in range(2):
+        plain.solve()
+        plain_ms.append(plain.get_stats().last_solve_ms)
+    pst = plain.get_stats()
+    plain_rate = pst.updates / (min(plain_ms) / 1000.0)
+    log("plain FP64 single source: %.1f ms, %.3e updates/s"
+        % (min(plain_ms), plain_rate))
+    del plain
+    del batch
+
+    # ---- CPU baseline (oracle, bounded sample) ------------------------------
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        rate, secs, upd = cpu_baseline(w, slow, w["sources"][picks[0]],
+                                       threads, args.cpu_iterations)
+        cpu = {"value": rate, "unit": "grid-pt updates/s", "cores": threads,
+               "kind": "port",
+               "sample": "1 of %d sources on the full %dx%dx%d grid, 8 init "
+                         "+ %dx8 refinement sweeps (%d updates, %.1f s), FP64 "
+                         "hyperplane order, OpenMP"
+                         % (len(w["sources"]), nx, ny, nz,
+                            args.cpu_iterations, upd, secs)}
+        log("cpu baseline %.3e updates/s on %d threads (%.1f s)"
+            % (rate, threads, secs))
+
+    result = {
+        "metric": "grid-pt updates/s & % HBM roofline, FP32-batched vs "
+                  "FP64 plain at <=1e-5 rel err",
+        "value": value,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered + Gaussian-field model)",
+        "config": {
+            "workload": "station-reciprocal tables: %d sources (surface "
+                        "stations) on one %dx%dx%d model, h=%g m, "
+                        "layers 1500-8000 m/s x (1+0.08 G), l=500 m, "
+                        "%d iterations, factored radius %d"
+                        % (len(w["sources"]), nx, ny, nz, h, w["n_sweeps"],
+                           w["radius"]),
+            "precision": "fp32 sweeps, fp64 near source, final iteration "
+                         "fp64 (polish)",
+            "updates_per_step": updates,
+            "updates_fp64_per_step": updates64,
+            "device_ms_per_step": float(np.mean(dev_ms)),
+            "max_rel_err_vs_fp64": max_err,
+            "rms_rel_err_vs_fp64": rms_err,
+            "error_sources": len(picks),
+            "fp64_plain_updates_per_s": plain_rate,
+            "fp64_plain_workload": "1 source, same grid and schedule, FP64",
+            "speedup_vs_fp64_plain": value / plain_rate,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -90,9 +90,12 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("updates", ctypes.c_int64),
                 ("updates_fp64", ctypes.c_int64),
                 ("kernel_launches", ctypes.c_int64),
+                ("kernel_launches_fp64", ctypes.c_int64),
                 ("iterations_run", ctypes.c_int32),
                 ("reserved", ctypes.c_int32),
-                ("last_solve_ms", ctypes.c_double)]
+                ("last_solve_ms", ctypes.c_double),
+                ("sweep_ms_fp32", ctypes.c_double),
+                ("sweep_ms_fp64", ctypes.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_

@@ -277,11 +277,33 @@ struct Engine3D
     DeviceBuffer<Problem3D> dProblems;
     DeviceBuffer<ColumnItem> dItems;
     std::vector<int> diagStart;
+    // Upper bound, over any source position, of the columns per diagonal
+    // that overlap one problem's factored box: the box spans at most
+    // ceil((2r+1)/8)+1 columns per axis.
+    int boxPerDiag = 0;
     hipStream_t stream = nullptr;
+    hipStream_t sideStream = nullptr;   // factored-box columns
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
     hipEvent_t evStart = nullptr, evStop = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    bool graphValid = false;
+    // segment events: [0,1] FP32 sweeps, [2,3] FP64 sweeps
+    hipEvent_t evSeg[4] = {nullptr, nullptr, nullptr, nullptr};
+    // The sweeps of one precision, captured once and replayed per solve.
+    struct Segment
+    {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        bool valid = false;
+        ekx_solve_stats st{};
+        void release()
+        {
+            if (exec) { (void) hipGraphExecDestroy(exec); }
+            if (graph) { (void) hipGraphDestroy(graph); }
+            exec = nullptr;
+            graph = nullptr;
+            valid = false;
+        }
+    };
+    Segment segment[2];   // [0] primary sweeps, [1] FP64 polish sweeps
     bool haveModel = false;
     bool haveField = false;
     bool fieldIsF64 = true;
@@ -292,16 +314,20 @@ struct Engine3D
         drop_graph();
         if (evStart) { (void) hipEventDestroy(evStart); }
         if (evStop) { (void) hipEventDestroy(evStop); }
+        if (evFork) { (void) hipEventDestroy(evFork); }
+        if (evJoin) { (void) hipEventDestroy(evJoin); }
+        if (sideStream) { (void) hipStreamDestroy(sideStream); }
+        for (hipEvent_t &e : evSeg)
+        {
+            if (e) { (void) hipEventDestroy(e); }
+        }
         if (stream) { (void) hipStreamDestroy(stream); }
     }
 
     void drop_graph()
     {
-        if (graphExec) { (void) hipGraphExecDestroy(graphExec); }
-        if (graph) { (void) hipGraphDestroy(graph); }
-        graphExec = nullptr;
-        graph = nullptr;
-        graphValid = false;
+        segment[0].release();
+        segment[1].release();
     }
 
     bool sweeps_f64() const
@@ -339,8 +365,15 @@ struct Engine3D
         if (!stream)
         {
             EKX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            EKX_HIP(hipStreamCreateWithFlags(&sideStream,
+                                             hipStreamNonBlocking));
             EKX_HIP(hipEventCreate(&evStart));
             EKX_HIP(hipEventCreate(&evStop));
+            EKX_HIP(hipEventCreateWithFlags(&evFork,
+                                            hipEventDisableTiming));
+            EKX_HIP(hipEventCreateWithFlags(&evJoin,
+                                            hipEventDisableTiming));
+            for (hipEvent_t &e : evSeg) { EKX_HIP(hipEventCreate(&e)); }
         }
         // model offsets (cells)
         modelOffset.assign(nModels, -1);
@@ -412,6 +445,20 @@ struct Engine3D
                 {
                     perDiag[bx + by].push_back(ColumnItem{i, bx, by, 0});
                 }
+            }
+        }
+        boxPerDiag = 0;
+        if (o.factored_radius >= 0)
+        {
+            int span = (2*o.factored_radius + 1 + kColumnTile - 1)/kColumnTile
+                     + 1;
+            for (int i = 0; i < n; ++i)
+            {
+                const ekx_geometry3d &g = geoms[i];
+                int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
+                int nby = (g.ny + kColumnTile - 1)/kColumnTile;
+                boxPerDiag = std::max(boxPerDiag,
+                                      std::min(span, std::min(nbx, nby)));
             }
         }
         std::vector<ColumnItem> flat;
@@ -499,6 +546,19 @@ struct Engine3D
             st.updates += upd;
             if (sizeof(TS) == 8) { st.updates_fp64 += upd; }
         }
+        int64_t launches = 0;
+        ekx_status rc = enqueue_sweep_launches<TS>(sweep, initialize, T, S,
+                                                   launches);
+        st.kernel_launches += launches;
+        if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
+        return rc;
+    }
+
+    template <typename TS>
+    ekx_status enqueue_sweep_launches(int sweep, bool initialize, TS *T,
+                                      const TS *S, int64_t &launches)
+    {
+        const int n = (int) problems.size();
         if (kernel() == EKX_KERNEL_HYPERPLANE)
         {
             for (int i = 0; i < n; ++i)
@@ -515,34 +575,49 @@ struct Engine3D
                     EKX_HIP(launch_hyperplane3d<TS>(
                         dProblems.ptr, i, sweep, initialize ? 1 : 0, L, kLo,
                         nPairs, T, S, stream));
-                    st.kernel_launches += 1;
+                    launches += 1;
                 }
             }
         }
         else
         {
+            // Per diagonal: the factored-box columns run on the side stream
+            // concurrently with the bulk (Cartesian) launch; both join
+            // before the next diagonal.
             const int nDiag = (int) diagStart.size() - 1;
             for (int d = 0; d < nDiag; ++d)
             {
                 int count = diagStart[d + 1] - diagStart[d];
+                if (boxPerDiag > 0)
+                {
+                    EKX_HIP(hipEventRecord(evFork, stream));
+                    EKX_HIP(hipStreamWaitEvent(sideStream, evFork, 0));
+                    EKX_HIP(launch_factored_columns3d<TS>(
+                        dProblems.ptr, n, sweep, initialize ? 1 : 0, d,
+                        boxPerDiag, T, S, sideStream));
+                    launches += 1;
+                }
                 EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
                                              dItems.ptr + diagStart[d], count,
-                                             sweep, initialize ? 1 : 0, T, S,
-                                             stream));
-                st.kernel_launches += 1;
+                                             sweep, initialize ? 1 : 0, d,
+                                             boxPerDiag, T, S, stream));
+                launches += 1;
+                if (boxPerDiag > 0)
+                {
+                    EKX_HIP(hipEventRecord(evJoin, sideStream));
+                    EKX_HIP(hipStreamWaitEvent(stream, evJoin, 0));
+                }
             }
         }
         return EKX_OK;
     }
 
     template <typename TS>
-    ekx_status enqueue_phase(TS *T, const TS *S, bool doInit, int iterations,
-                             ekx_solve_stats &st)
+    ekx_status enqueue_phase_sweeps(TS *T, const TS *S, bool doInit,
+                                    int iterations, ekx_solve_stats &st)
     {
         if (doInit)
         {
-            EKX_HIP(launch_init3d<TS>(dProblems.ptr, (int) problems.size(),
-                                      maxNodes, T, stream));
             if (opt.schedule == EKX_SCHEDULE_SHIPPED)
             {
                 // src/solver3d.cpp:451-463 (FSM); level set: nothing.
@@ -569,27 +644,53 @@ struct Engine3D
         return EKX_OK;
     }
 
-    // Issues every launch of one solve on the stream.
-    ekx_status enqueue_solve(ekx_solve_stats &st)
+    // Runs the sweeps of one precision segment: replayed from its captured
+    // graph (captured on first use) or issued directly.
+    template <typename TS>
+    ekx_status run_segment(int k, TS *T, const TS *S, bool doInit,
+                           int iterations, ekx_solve_stats &st)
     {
-        st = ekx_solve_stats{};
-        const int nIt = opt.number_of_sweeps;
-        if (sweeps_f64())
+        ekx_solve_stats local{};
+        if (opt.use_graph)
         {
-            EKX_CHECK(enqueue_phase<double>(T64.ptr, S64.ptr, true, nIt, st));
+            Segment &g = segment[k];
+            if (!g.valid)
+            {
+                g.release();
+                EKX_HIP(hipStreamBeginCapture(stream,
+                                              hipStreamCaptureModeThreadLocal));
+                ekx_status s = enqueue_phase_sweeps<TS>(T, S, doInit,
+                                                        iterations, local);
+                hipGraph_t captured = nullptr;
+                hipError_t e = hipStreamEndCapture(stream, &captured);
+                if (s != EKX_OK)
+                {
+                    if (captured) { (void) hipGraphDestroy(captured); }
+                    return s;
+                }
+                EKX_HIP(e);
+                g.graph = captured;
+                if (local.kernel_launches > 0)
+                {
+                    EKX_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr,
+                                                nullptr, 0));
+                }
+                g.st = local;
+                g.valid = true;
+            }
+            local = g.st;
+            if (g.exec) { EKX_HIP(hipGraphLaunch(g.exec, stream)); }
         }
         else
         {
-            int itF32 = polish() ? nIt - 1 : nIt;
-            if (opt.schedule == EKX_SCHEDULE_SHIPPED) { itF32 = 0; }
-            EKX_CHECK(enqueue_phase<float>(T32.ptr, S32.ptr, true, itF32, st));
-            if (polish())
-            {
-                EKX_HIP(launch_convert(totalNodes, T32.ptr, T64.ptr, stream));
-                EKX_CHECK(enqueue_phase<double>(T64.ptr, S64.ptr, false, 1,
-                                                st));
-            }
+            EKX_CHECK(enqueue_phase_sweeps<TS>(T, S, doInit, iterations,
+                                               local));
         }
+        st.updates += local.updates;
+        st.updates_fp64 += local.updates_fp64;
+        st.kernel_launches += local.kernel_launches;
+        st.kernel_launches_fp64 += local.kernel_launches_fp64;
+        st.iterations_run += local.iterations_run;
         return EKX_OK;
     }
 
@@ -609,47 +710,52 @@ struct Engine3D
         haveField = false;
         EKX_CHECK(prepare_sources());
         ekx_solve_stats st{};
-        if (opt.use_graph)
+        const int nIt = opt.number_of_sweeps;
+        const int nProb = (int) problems.size();
+        bool polished = false;
+        EKX_HIP(hipEventRecord(evStart, stream));
+        if (sweeps_f64())
         {
-            if (!graphValid)
-            {
-                drop_graph();
-                EKX_HIP(hipStreamBeginCapture(stream,
-                                              hipStreamCaptureModeThreadLocal));
-                ekx_status s = enqueue_solve(st);
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(stream, &g);
-                if (s != EKX_OK)
-                {
-                    if (g) { (void) hipGraphDestroy(g); }
-                    return s;
-                }
-                EKX_HIP(e);
-                graph = g;
-                EKX_HIP(hipGraphInstantiate(&graphExec, graph, nullptr,
-                                            nullptr, 0));
-                graphValid = true;
-                stats = st;
-            }
-            else
-            {
-                st = stats;
-            }
-            EKX_HIP(hipEventRecord(evStart, stream));
-            EKX_HIP(hipGraphLaunch(graphExec, stream));
-            EKX_HIP(hipEventRecord(evStop, stream));
+            EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, maxNodes,
+                                          T64.ptr, stream));
+            EKX_HIP(hipEventRecord(evSeg[0], stream));
+            EKX_CHECK(run_segment<double>(0, T64.ptr, S64.ptr, true, nIt,
+                                          st));
+            EKX_HIP(hipEventRecord(evSeg[1], stream));
         }
         else
         {
-            EKX_HIP(hipEventRecord(evStart, stream));
-            EKX_CHECK(enqueue_solve(st));
-            EKX_HIP(hipEventRecord(evStop, stream));
+            int itF32 = polish() ? nIt - 1 : nIt;
+            EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, maxNodes,
+                                         T32.ptr, stream));
+            EKX_HIP(hipEventRecord(evSeg[0], stream));
+            EKX_CHECK(run_segment<float>(0, T32.ptr, S32.ptr, true, itF32,
+                                         st));
+            EKX_HIP(hipEventRecord(evSeg[1], stream));
+            if (polish())
+            {
+                EKX_HIP(launch_convert(totalNodes, T32.ptr, T64.ptr, stream));
+                EKX_HIP(hipEventRecord(evSeg[2], stream));
+                EKX_CHECK(run_segment<double>(1, T64.ptr, S64.ptr, false, 1,
+                                              st));
+                EKX_HIP(hipEventRecord(evSeg[3], stream));
+                polished = true;
+            }
         }
+        EKX_HIP(hipEventRecord(evStop, stream));
         EKX_HIP(hipEventSynchronize(evStop));
         EKX_HIP(hipGetLastError());
         float ms = 0;
         EKX_HIP(hipEventElapsedTime(&ms, evStart, evStop));
         st.last_solve_ms = ms;
+        EKX_HIP(hipEventElapsedTime(&ms, evSeg[0], evSeg[1]));
+        if (sweeps_f64()) { st.sweep_ms_fp64 = ms; }
+        else { st.sweep_ms_fp32 = ms; }
+        if (polished)
+        {
+            EKX_HIP(hipEventElapsedTime(&ms, evSeg[2], evSeg[3]));
+            st.sweep_ms_fp64 = ms;
+        }
         stats = st;
         haveField = true;
         return EKX_OK;

@@ -5,12 +5,13 @@
 namespace ekx
 {
 
-// Waves per workgroup of the column kernel: LDS per wave is
-// 16 slots x 2 x 84 values, so FP32 blocks of 4 waves and FP64 blocks of 2
-// waves both use 43 KB.
+// Waves per workgroup of the column kernel.  Each wave owns 17 LDS planes
+// of 2 x 84 values (11.4 KB in FP32, 22.8 KB in FP64), so LDS (160 KB per
+// CU) bounds residency at 14 FP32 or 7 FP64 waves per CU; small blocks let
+// the dispatcher reach that bound.
 template <typename TS> constexpr int column_waves()
 {
-    return sizeof(TS) == 4 ? 4 : 2;
+    return sizeof(TS) == 4 ? 2 : 1;
 }
 
 template <typename TS>
@@ -21,11 +22,21 @@ hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,
                                int nPairs, TS *T, const TS *S,
                                hipStream_t stream);
+// One column diagonal, Cartesian operators: the diagonal's items, except
+// the columns overlapping a factored box (boxPerDiag > 0).
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
-                            int initialize, TS *T, const TS *S,
-                            hipStream_t stream);
+                            int initialize, int diagonal, int boxPerDiag,
+                            TS *T, const TS *S, hipStream_t stream);
+// The same diagonal's columns that overlap a factored box: up to
+// boxPerDiag per problem, full local update.  Independent of the launch
+// above (disjoint columns), so the two may run concurrently.
+template <typename TS>
+hipError_t launch_factored_columns3d(const Problem3D *dProblems,
+                                     int nProblems, int sweep, int initialize,
+                                     int diagonal, int boxPerDiag, TS *T,
+                                     const TS *S, hipStream_t stream);
 hipError_t launch_nodal_to_cell3d(int nx, int ny, int nz, const double *v,
                                   int ordering, double *slow,
                                   hipStream_t stream);

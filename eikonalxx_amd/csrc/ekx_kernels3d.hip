@@ -35,6 +35,34 @@ __device__ __forceinline__ Octant sweep_octant(const Problem3D &p, int sweep,
     return o;
 }
 
+// Columns (8 x 8 footprints in the permuted frame) that overlap the
+// factored-eikonal box |i - iSrc| <= radius (solverUtilities3d.hpp:296-298).
+// Empty (lo > hi) when the radius is negative.
+struct BoxColumns
+{
+    int bxLo, bxHi, byLo, byHi;
+};
+
+__device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
+                                                  int sweep)
+{
+    BoxColumns c{1, 0, 1, 0};
+    if (p.radius < 0) { return c; }
+    const int negX = sweep & 1, negY = (sweep >> 1) & 1;
+    int xLo = max(p.srcCellX - p.radius, 0);
+    int xHi = min(p.srcCellX + p.radius, p.nx - 1);
+    int yLo = max(p.srcCellY - p.radius, 0);
+    int yHi = min(p.srcCellY + p.radius, p.ny - 1);
+    if (xLo > xHi || yLo > yHi) { return c; }
+    int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
+    int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
+    c.bxLo = jxLo/kColumnTile;
+    c.bxHi = jxHi/kColumnTile;
+    c.byLo = jyLo/kColumnTile;
+    c.byHi = jyHi/kColumnTile;
+    return c;
+}
+
 // ---------------------------------------------------------------------------
 // Source initialisation (Solver3DImpl::initializeTravelTimes,
 // src/solver3d.cpp:21-84): HUGE everywhere, the analytic time at the 8
@@ -155,28 +183,67 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
 // diagonal never read or write each other's nodes; their halos belong to
 // the diagonals before.
 // ---------------------------------------------------------------------------
-template <typename TS, int WAVES, int DL>
+//
+// Two instances per diagonal: FACTORED = false runs the Cartesian operators
+// only and skips the columns that overlap the factored box around the
+// source; FACTORED = true runs the full local update on exactly those
+// columns (at most boxPerDiag per problem and diagonal), with items
+// derived from the problem's source instead of the item list.  Keeping the
+// factored arithmetic out of the bulk kernel keeps its register count, and
+// so its occupancy, at that of the Cartesian update.
+// ---------------------------------------------------------------------------
+template <typename TS, int WAVES, int DL, bool FACTORED>
 __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
                 const ColumnItem *__restrict__ items, int nItems, int sweep,
-                int initialize, TS *__restrict__ T,
-                const TS *__restrict__ S)
+                int initialize, int diagonal, int boxPerDiag,
+                TS *__restrict__ T, const TS *__restrict__ S)
 {
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
-    __shared__ TS ring[WAVES][kColumnRing][2][kSlot];
+    // kColumnRing slots plus one spare that absorbs the stores of the
+    // padding steps after the last plane.
+    __shared__ TS ring[WAVES][kColumnRing + 1][2][kSlot];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int itemIndex = blockIdx.x*WAVES + wave;
     if (itemIndex >= nItems) { return; }
-    const ColumnItem item = items[itemIndex];
-    const int pIndex = __builtin_amdgcn_readfirstlane(item.problem);
+    int pIndex, bx, by;
+    if (FACTORED)
+    {
+        // item = (problem, k): the k-th box column on this diagonal
+        pIndex = itemIndex/boxPerDiag;
+        const int k = itemIndex - pIndex*boxPerDiag;
+        const BoxColumns bc = box_columns(problems[pIndex], sweep);
+        const int lo = max(bc.bxLo, diagonal - bc.byHi);
+        const int hi = min(bc.bxHi, diagonal - bc.byLo);
+        if (lo + k > hi) { return; }
+        bx = lo + k;
+        by = diagonal - bx;
+    }
+    else
+    {
+        const ColumnItem item = items[itemIndex];
+        pIndex = item.problem;
+        bx = item.bx;
+        by = item.by;
+    }
+    pIndex = __builtin_amdgcn_readfirstlane(pIndex);
     const Problem3D &p = problems[pIndex];
+    if (!FACTORED)
+    {
+        const BoxColumns bc = box_columns(p, sweep);
+        if (bx >= bc.bxLo && bx <= bc.bxHi && by >= bc.byLo &&
+            by <= bc.byHi)
+        {
+            return;   // handled by the FACTORED instance
+        }
+    }
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const int ncx = nx - 1, ncy = ny - 1, ncz = nz - 1;
     const int negX = sweep & 1, negY = (sweep >> 1) & 1, negZ = (sweep >> 2) & 1;
     const Octant o = sweep_octant(p, sweep, initialize);
-    const int X0 = __builtin_amdgcn_readfirstlane(item.bx)*kColumnTile;
-    const int Y0 = __builtin_amdgcn_readfirstlane(item.by)*kColumnTile;
+    const int X0 = __builtin_amdgcn_readfirstlane(bx)*kColumnTile;
+    const int Y0 = __builtin_amdgcn_readfirstlane(by)*kColumnTile;
     if (X0 + kColumnTile - 1 < o.jx0 || Y0 + kColumnTile - 1 < o.jy0)
     {
         return;   // column entirely outside the source octant
@@ -195,29 +262,49 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int64_t nxy = (int64_t) nx*ny;
     const int64_t ncxy = (int64_t) ncx*ncy;
 
-    // Ring elements this lane loads: e0 = lane, e1 = lane + 64 (< 81).
-    const bool has1 = lane + 64 < kColumnPlane;
-    int tXY0 = -1, tXY1 = -1, sXY0, sXY1;
+    // LDS plane layout (index of local coordinate (c, r), c = jx - X0 + 1,
+    // r = jy - Y0 + 1, both in 0..8): the 8 x 8 core row-major with stride
+    // 8 at 0..63 (conflict-free for the lanes' own nodes), the halo row
+    // r = 0 at 64..72, the halo column c = 0 at 73..80.  Cells use the same
+    // layout with c, r the cell's local indices.
+    auto lidx = [](int c, int r)
     {
-        auto setup = [&](int e, int &tXY, int &sXY)
+        return (r >= 1 && c >= 1) ? (r - 1)*kColumnTile + (c - 1)
+             : (r == 0 ? kColumnTile*kColumnTile + c
+                       : kColumnTile*kColumnTile + kColumnHalo + (r - 1));
+    };
+    // Ring elements this lane loads: e0 = lane, e1 = lane + 64 (< 81).
+    // Every prefetch load is unconditional from a clamped, in-bounds
+    // address (lanes without a second element reload their first one);
+    // nodes outside the grid are replaced by HUGE when the plane enters the
+    // ring.  Conditional loads would make the compiler wait for them at the
+    // merge point and serialise the prefetch.
+    const bool has1 = lane + 64 < kColumnPlane;
+    const int dst0 = lidx(lane%kColumnHalo, lane/kColumnHalo);
+    const int dst1 = has1 ? lidx((lane + 64)%kColumnHalo,
+                                 (lane + 64)/kColumnHalo)
+                          : kColumnPlane;             // padding if unused
+    int tXY0, tXY1, sXY0, sXY1;
+    bool tOk0, tOk1;
+    {
+        auto setup = [&](int e, int &tXY, int &sXY, bool &ok)
         {
             int c = e%kColumnHalo, r = e/kColumnHalo;
             int gx = X0 - 1 + c, gy = Y0 - 1 + r;
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-            {
-                int ax = negX ? nx - 1 - gx : gx;
-                int ay = negY ? ny - 1 - gy : gy;
-                tXY = ay*nx + ax;
-            }
+            ok = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+            int cgx = min(max(gx, 0), nx - 1);
+            int cgy = min(max(gy, 0), ny - 1);
+            int ax = negX ? nx - 1 - cgx : cgx;
+            int ay = negY ? ny - 1 - cgy : cgy;
+            tXY = ay*nx + ax;
             int pcx = min(max(gx, 0), ncx - 1);
             int pcy = min(max(gy, 0), ncy - 1);
             int cx = negX ? ncx - 1 - pcx : pcx;
             int cy = negY ? ncy - 1 - pcy : pcy;
             sXY = cy*ncx + cx;
         };
-        setup(lane, tXY0, sXY0);
-        sXY1 = sXY0;
-        if (has1) { setup(lane + 64, tXY1, sXY1); }
+        setup(lane, tXY0, sXY0, tOk0);
+        setup(has1 ? lane + 64 : lane, tXY1, sXY1, tOk1);
     }
     const TS huge = (TS) kHuge;
     auto loadPlane = [&](int q, TS &vT0, TS &vT1, TS &vS0, TS &vS1)
@@ -228,27 +315,39 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         const int cz = negZ ? ncz - 1 - pcz : pcz;
         const TS *Tz = Tp + (int64_t) az*nxy;
         const TS *Sz = Sp + (int64_t) cz*ncxy;
-        vT0 = tXY0 >= 0 ? Tz[tXY0] : huge;
+        vT0 = Tz[tXY0];
         vS0 = Sz[sXY0];
-        vT1 = huge;
-        vS1 = vS0;
-        if (has1)
-        {
-            vT1 = tXY1 >= 0 ? Tz[tXY1] : huge;
-            vS1 = Sz[sXY1];
-        }
+        vT1 = Tz[tXY1];
+        vS1 = Sz[sXY1];
     };
 
     TS bT0[DL], bT1[DL], bS0[DL], bS1[DL];
+    // Prologue: buffer u holds plane u.  The scheduling barriers keep the
+    // buffers in issue order, so the first use of buffer u waits only for
+    // the loads issued before it (as in the steady state).
 #pragma unroll
     for (int u = 0; u < DL; ++u)
     {
-        if (u < nPlanes) { loadPlane(u, bT0[u], bT1[u], bS0[u], bS1[u]); }
+        loadPlane(min(u, nPlanes - 1), bT0[u], bT1[u], bS0[u], bS1[u]);
+        __builtin_amdgcn_sched_barrier(0);
     }
     TS pt4 = huge, pt5 = huge, pt6 = huge, pt7 = huge;
     TS ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
-    const int own = (b + 1)*kColumnHalo + a + 1;
-    const int nSteps = nPlanes + kColumnSkew + 1;
+    // The node is local (a+1, b+1).  Its neighbours (x-1, y-1, xy-1) and
+    // its 4 cells of one plane share these four LDS offsets.
+    const int oA = lidx(a, b), oB = lidx(a + 1, b), oC = lidx(a, b + 1);
+    const int oD = lidx(a + 1, b + 1);
+    // write-back lane: itself, or the nearest in-grid lane of its row/column
+    const int wbA = min(a, nx - 1 - X0), wbB = min(b, ny - 1 - Y0);
+    const int wbOwn = lidx(wbA + 1, wbB + 1);
+    const int wbX = negX ? nx - 1 - (X0 + wbA) : X0 + wbA;
+    const int wbY = negY ? ny - 1 - (Y0 + wbB) : Y0 + wbB;
+    const int64_t wbXY = (int64_t) wbY*nx + wbX;
+    // Steps: every plane passes every lane and is written back; rounded up
+    // to whole groups of DL so the unrolled loop body has no exit (extra
+    // steps only repeat idempotent work: clamped loads, no computation, and
+    // a re-store of the last plane's final values).
+    const int nSteps = ((nPlanes + kColumnSkew + 1 + DL - 1)/DL)*DL;
     TS(*myRing)[2][kSlot] = ring[wave];
     for (int t0 = 0; t0 < nSteps; t0 += DL)
     {
@@ -256,50 +355,68 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         for (int u = 0; u < DL; ++u)
         {
             const int tc = t0 + u;
-            if (tc >= nSteps) { break; }
-            // (a) the plane prefetched DL steps ago enters the ring
-            if (tc < nPlanes)
+            // (a) the plane prefetched DL steps ago enters the ring.  Branch
+            // free: past the last plane it goes to the spare slot, and lanes
+            // without a second element write the slot's padding.
             {
-                TS *dT = myRing[tc & (kColumnRing - 1)][0];
-                TS *dS = myRing[tc & (kColumnRing - 1)][1];
-                dT[lane] = bT0[u];
-                dS[lane] = bS0[u];
-                if (has1)
-                {
-                    dT[lane + 64] = bT1[u];
-                    dS[lane + 64] = bS1[u];
-                }
+                const int slot = tc < nPlanes ? (tc & (kColumnRing - 1))
+                                              : kColumnRing;
+                TS *dT = myRing[slot][0];
+                TS *dS = myRing[slot][1];
+                dT[dst0] = tOk0 ? bT0[u] : huge;
+                dS[dst0] = bS0[u];
+                dT[dst1] = tOk1 ? bT1[u] : huge;
+                dS[dst1] = bS1[u];
             }
-            // (b) prefetch plane tc + DL into the freed registers
-            if (tc + DL < nPlanes)
-            {
-                loadPlane(tc + DL, bT0[u], bT1[u], bS0[u], bS1[u]);
-            }
+            // (b) prefetch plane tc + DL into the freed registers (past the
+            // last plane the load repeats it and is never stored)
+            loadPlane(min(tc + DL, nPlanes - 1), bT0[u], bT1[u], bS0[u],
+                      bS1[u]);
+            // (d, read) the plane the last lane finished a step ago; read
+            // with the neighbour values below, stored after the update
+            const int w = min(max(tc - (kColumnSkew + 1), 0), nPlanes - 1);
+            const TS wbValue = myRing[w & (kColumnRing - 1)][0][wbOwn];
             // (c) every lane advances its node on plane tc - (a + b)
             const int q = tc - (a + b);
             if (valid && q >= 0 && q < nPlanes)
             {
                 TS *sT = myRing[q & (kColumnRing - 1)][0];
                 const TS *sS = myRing[q & (kColumnRing - 1)][1];
-                TS tOwn = sT[own];
-                const TS t1 = sT[own - 1];
-                const TS t2 = sT[own - 1 - kColumnHalo];
-                const TS t3 = sT[own - kColumnHalo];
-                const TS c00 = sS[b*kColumnHalo + a];
-                const TS c10 = sS[b*kColumnHalo + a + 1];
-                const TS c01 = sS[(b + 1)*kColumnHalo + a];
-                const TS c11 = sS[(b + 1)*kColumnHalo + a + 1];
+                TS tOwn = sT[oD];
+                const TS t1 = sT[oC];
+                const TS t2 = sT[oA];
+                const TS t3 = sT[oB];
+                const TS c00 = sS[oA];
+                const TS c10 = sS[oB];
+                const TS c01 = sS[oC];
+                const TS c11 = sS[oD];
                 if (q > 0 && upd)
                 {
                     const int jz = P0 + q;
                     const int iz = negZ ? nz - 1 - jz : jz;
-                    const TS tUpd = local_update3d<TS>(
-                        p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00, c10,
-                        c01, t1, t2, t3, pt4, pt5, pt6, pt7);
+                    TS tUpd;
+                    if (FACTORED)
+                    {
+                        tUpd = local_update3d<TS>(
+                            p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00,
+                            c10, c01, t1, t2, t3, pt4, pt5, pt6, pt7);
+                    }
+                    else if (p.uniform)
+                    {
+                        tUpd = cartesian_uniform3d<TS>(
+                            (TS) p.dx, ps00, ps10, ps11, ps01, c00, c10, c01,
+                            t1, t2, t3, pt4, pt5, pt6, pt7);
+                    }
+                    else
+                    {
+                        tUpd = cartesian_nonuniform3d<TS>(
+                            p, ps00, ps10, ps11, ps01, c00, c10, c01, t1, t2,
+                            t3, pt4, pt5, pt6, pt7);
+                    }
                     if (tUpd < tOwn)
                     {
                         tOwn = tUpd;
-                        sT[own] = tUpd;
+                        sT[oD] = tUpd;
                     }
                 }
                 pt4 = tOwn;
@@ -311,14 +428,15 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 ps01 = c01;
                 ps11 = c11;
             }
-            // (d) the plane the last lane finished a step ago goes home
-            const int w = tc - (kColumnSkew + 1);
-            if (w >= 1 && w < nPlanes && upd)
+            // (d, store) the finished plane goes home.  The store is issued
+            // at every step (plane index clamped to the ring's valid range,
+            // unchanged nodes store their own value, lanes past the grid
+            // edge repeat an edge lane's store), so the prefetch loads above
+            // always have the same number of memory operations behind them.
             {
                 const int jz = P0 + w;
                 const int az = negZ ? nz - 1 - jz : jz;
-                Tp[(int64_t) az*nxy + (int64_t) iy*nx + ix]
-                    = myRing[w & (kColumnRing - 1)][0][own];
+                Tp[(int64_t) az*nxy + wbXY] = wbValue;
             }
             // LDS traffic of this step must not be reordered with the next
             // step's: other lanes of the wave consume it.
@@ -446,15 +564,30 @@ hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
-                            int initialize, TS *T, const TS *S,
-                            hipStream_t stream)
+                            int initialize, int diagonal, int boxPerDiag,
+                            TS *T, const TS *S, hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
     constexpr int W = column_waves<TS>();
-    hipLaunchKernelGGL((sweep_columns3d<TS, W, 4>),
+    hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>),
                        dim3((unsigned) ((nItems + W - 1)/W)), dim3(64*W), 0,
                        stream, dProblems, dItems, nItems, sweep, initialize,
-                       T, S);
+                       diagonal, boxPerDiag, T, S);
+    return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t launch_factored_columns3d(const Problem3D *dProblems,
+                                     int nProblems, int sweep, int initialize,
+                                     int diagonal, int boxPerDiag, TS *T,
+                                     const TS *S, hipStream_t stream)
+{
+    if (boxPerDiag <= 0) { return hipSuccess; }
+    int n = nProblems*boxPerDiag;
+    hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true>), dim3((unsigned) n),
+                       dim3(64), 0, stream, dProblems,
+                       (const ColumnItem *) nullptr, n, sweep, initialize,
+                       diagonal, boxPerDiag, T, S);
     return hipGetLastError();
 }
 
@@ -510,11 +643,20 @@ template hipError_t launch_hyperplane3d<double>(const Problem3D *, int, int,
                                                 const double *, hipStream_t);
 template hipError_t launch_columns3d<float>(const Problem3D *,
                                             const ColumnItem *, int, int, int,
-                                            float *, const float *,
+                                            int, int, float *, const float *,
                                             hipStream_t);
 template hipError_t launch_columns3d<double>(const Problem3D *,
                                              const ColumnItem *, int, int,
-                                             int, double *, const double *,
-                                             hipStream_t);
+                                             int, int, int, double *,
+                                             const double *, hipStream_t);
+template hipError_t launch_factored_columns3d<float>(const Problem3D *, int,
+                                                     int, int, int, int,
+                                                     float *, const float *,
+                                                     hipStream_t);
+template hipError_t launch_factored_columns3d<double>(const Problem3D *, int,
+                                                      int, int, int, int,
+                                                      double *,
+                                                      const double *,
+                                                      hipStream_t);
 
 } // namespace ekx

@@ -1,0 +1,108 @@
+"""Seeded synthetic velocity models and source sets for the benchmark
+workloads (host-side input preparation; the solve itself runs on the GPU).
+
+Every generator is deterministic in its seed.  Velocities are nodal
+(nx*ny*nz, x fastest); ``cell_slowness`` converts them to the cell slowness
+the solver consumes with the reference's 8-node average
+(src/model3d.cpp:101-107: s = 8 / sum of the 8 corner velocities).
+"""
+import numpy as np
+
+try:
+    from scipy.ndimage import gaussian_filter
+except ImportError:  # pragma: no cover - scipy is part of the image
+    gaussian_filter = None
+
+
+def cell_slowness(v, nx, ny, nz):
+    """Nodal velocity (z, y, x order flattened) -> cell slowness."""
+    v = np.asarray(v, dtype=np.float64).reshape(nz, ny, nx)
+    s1 = v[:-1, :-1, :-1] + v[:-1, :-1, 1:]
+    s2 = v[:-1, 1:, :-1] + v[:-1, 1:, 1:]
+    s3 = v[1:, :-1, :-1] + v[1:, :-1, 1:]
+    s4 = v[1:, 1:, :-1] + v[1:, 1:, 1:]
+    return (8.0 / (s1 + s2 + s3 + s4)).reshape(-1)
+
+
+def gaussian_field(shape, h, corr_length, seed):
+    """Unit-variance Gaussian random field with covariance ~exp(-r^2/l^2):
+    white noise convolved with a separable Gaussian of sigma = l/2,
+    renormalised to unit variance."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal(size=shape)
+    sigma = corr_length / (2.0 * h)
+    if gaussian_filter is None:
+        raise RuntimeError("scipy is required for the Gaussian field")
+    g = gaussian_filter(g, sigma=sigma, mode="reflect")
+    g -= g.mean()
+    g /= g.std()
+    return g
+
+
+def layered_gaussian_velocity(nx, ny, nz, h, n_layers, vmin, vmax,
+                              amplitude, corr_length, seed):
+    """n_layers equal-thickness layers with velocities U[vmin, vmax] sorted
+    to increase with depth, times (1 + amplitude * G)."""
+    rng = np.random.default_rng(seed)
+    layer_v = np.sort(rng.uniform(vmin, vmax, size=n_layers))
+    iz = np.arange(nz)
+    layer_of = np.minimum((iz * n_layers) // nz, n_layers - 1)
+    v_z = layer_v[layer_of]
+    G = gaussian_field((nz, ny, nx), h, corr_length, seed + 1)
+    v = v_z[:, None, None] * (1.0 + amplitude * G)
+    return v.reshape(-1), layer_v
+
+
+def config1_2d():
+    """2D plain FSM: 201 x 201, h = 10 m, v = 5000 m/s, source
+    (1005, 995) m, factored radius 3, 2 iterations."""
+    return dict(nx=201, nz=201, h=10.0, velocity=5000.0,
+                source=(1005.0, 995.0), n_sweeps=2, radius=3)
+
+
+def config2_gradient(n=129, h=10.0):
+    """3D plain: 129^3, h = 10 m, v(z) = 3000 + 0.5 z, source
+    (640, 640, 200) m, radius 3, 3 iterations."""
+    z = np.arange(n) * h
+    v = np.repeat((3000.0 + 0.5 * z)[:, None], n * n, axis=1).reshape(-1)
+    return dict(nx=n, ny=n, nz=n, h=h, velocity=v,
+                v_top=3000.0, v_bottom=3000.0 + 0.5 * (n - 1) * h,
+                source=(640.0, 640.0, 200.0), n_sweeps=3, radius=3)
+
+
+def config3_mixed(nx=257, ny=257, nz=193, h=25.0, seed=3003):
+    """3D mixed precision: 12 layers U[2000, 7000] times (1 + 0.05 G),
+    l = 250 m; one interior source; radius 5."""
+    v, _ = layered_gaussian_velocity(nx, ny, nz, h, 12, 2000.0, 7000.0,
+                                     0.05, 250.0, seed)
+    rng = np.random.default_rng(seed + 1)
+    src = tuple(float(rng.uniform(5 * h, (n - 6) * h))
+                for n in (nx, ny, nz))
+    return dict(nx=nx, ny=ny, nz=nz, h=h, velocity=v, source=src,
+                n_sweeps=3, radius=5)
+
+
+def config4_problem(i, h=20.0, lo=96, hi=320, seed0=4004):
+    """Problem i of the uneven batch: dims U{lo..hi} per axis, layered
+    Gaussian model as config 3, one seeded interior source."""
+    rng = np.random.default_rng(seed0 + i)
+    nx, ny, nz = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+    v, _ = layered_gaussian_velocity(nx, ny, nz, h, 12, 2000.0, 7000.0,
+                                     0.05, 250.0, seed0 + 1000 + i)
+    src = tuple(float(rng.uniform(2 * h, (n - 3) * h)) for n in (nx, ny, nz))
+    return dict(nx=nx, ny=ny, nz=nz, h=h, velocity=v, source=src)
+
+
+def config5_tables(nx=385, ny=385, nz=193, h=50.0, n_sources=256,
+                   seed=5005):
+    """Station-reciprocal tables: one shared model, layers U[1500, 8000]
+    (12 layers) times (1 + 0.08 G), l = 500 m; n_sources stations at
+    seeded surface points (z = 0) used as sources; 4 iterations."""
+    v, _ = layered_gaussian_velocity(nx, ny, nz, h, 12, 1500.0, 8000.0,
+                                     0.08, 500.0, seed)
+    rng = np.random.default_rng(seed + 7)
+    xs = rng.uniform(0.0, (nx - 1) * h, size=n_sources)
+    ys = rng.uniform(0.0, (ny - 1) * h, size=n_sources)
+    sources = [(float(x), float(y), 0.0) for x, y in zip(xs, ys)]
+    return dict(nx=nx, ny=ny, nz=nz, h=h, velocity=v, sources=sources,
+                n_sweeps=4, radius=3)

@@ -122,12 +122,16 @@ typedef struct
 /* Work and timing of the last solve on a handle. */
 typedef struct
 {
-    int64_t updates;          /* local-solver evaluations */
-    int64_t updates_fp64;     /* of which evaluated in FP64 sweeps        */
-    int64_t kernel_launches;  /* sweep kernel launches                    */
-    int32_t iterations_run;   /* refinement iterations actually run       */
+    int64_t updates;              /* local-solver evaluations: every node
+                                     inside a sweep's update range        */
+    int64_t updates_fp64;         /* of which evaluated in FP64 sweeps     */
+    int64_t kernel_launches;      /* sweep kernel launches                 */
+    int64_t kernel_launches_fp64; /* of which FP64 sweep launches          */
+    int32_t iterations_run;       /* refinement iterations actually run    */
     int32_t reserved;
-    double last_solve_ms;     /* device time of the last solve (events)   */
+    double last_solve_ms;         /* device time of the whole last solve   */
+    double sweep_ms_fp32;         /* device time of the FP32 sweep launches */
+    double sweep_ms_fp64;         /* device time of the FP64 sweep launches */
 } ekx_solve_stats;
 
 /* ---------------------------------------------------------------------- */

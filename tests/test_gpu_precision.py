@@ -1,0 +1,95 @@
+"""FP32 sweeps (FP64 near the source) and FP32 with the final iteration in
+FP64 (polish), against the FP64 GPU solve, which itself equals the oracle.
+
+Tolerance of the contract: max relative error |dT| / max(T64, s_src h)
+<= 1e-5 for the polished solve.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(t, t64, s_src, h):
+    r = np.abs(t - t64) / np.maximum(t64, s_src * h)
+    return r.max(), np.sqrt(np.mean(r * r))
+
+
+@pytest.fixture(scope="module")
+def mixed_case():
+    from eikonalxx_amd import workloads
+    w = workloads.config3_mixed(nx=97, ny=89, nz=73, h=25.0)
+    slow = workloads.cell_slowness(w["velocity"], w["nx"], w["ny"], w["nz"])
+    return w, slow
+
+
+def _solve(ekx, w, slow, precision, kernel=None, n_sweeps=None):
+    opts = ekx.SolverOptions(number_of_sweeps=n_sweeps or w["n_sweeps"],
+                             factored_radius=w["radius"],
+                             precision=precision,
+                             kernel=kernel or ekx.Kernel.AUTO)
+    s = ekx.Solver3D()
+    s.initialize(opts, ekx.Geometry3D(w["nx"], w["ny"], w["nz"], w["h"]))
+    s.set_slowness(slow)
+    s.set_source(w["source"])
+    s.solve()
+    return s
+
+
+def test_polish_within_1e5(ekx, mixed_case):
+    w, slow = mixed_case
+    t64 = _solve(ekx, w, slow, ekx.Precision.FP64).get_travel_time_field()
+    sp = _solve(ekx, w, slow, ekx.Precision.FP32_POLISH)
+    tp = sp.get_travel_time_field()
+    off, cell, idx = sp.get_source_cell()
+    s_src = slow[idx]
+    mx, rms = _rel(tp, t64, s_src, w["h"])
+    st = sp.get_stats()
+    print("polish: max %.3e rms %.3e" % (mx, rms), st.as_dict())
+    assert mx <= 1e-5
+    assert st.updates_fp64 > 0 and st.updates_fp64 < st.updates
+
+
+def test_fp32_error_reported(ekx, mixed_case):
+    w, slow = mixed_case
+    t64 = _solve(ekx, w, slow, ekx.Precision.FP64).get_travel_time_field()
+    s32 = _solve(ekx, w, slow, ekx.Precision.FP32)
+    t32 = s32.get_travel_time_field()
+    f32 = s32.get_travel_time_field_f32()
+    assert np.array_equal(t32, f32.astype(np.float64))
+    _, _, idx = s32.get_source_cell()
+    mx, rms = _rel(t32, t64, slow[idx], w["h"])
+    print("fp32 only: max %.3e rms %.3e" % (mx, rms))
+    assert mx <= 1e-4
+    assert s32.get_stats().updates_fp64 == 0
+
+
+def test_fp32_kernels_agree(ekx, mixed_case):
+    """Both kernel families run the same FP32 arithmetic in a topological
+    order, so their FP32 fields are identical."""
+    w, slow = mixed_case
+    a = _solve(ekx, w, slow, ekx.Precision.FP32, ekx.Kernel.COLUMN, 1)
+    b = _solve(ekx, w, slow, ekx.Precision.FP32, ekx.Kernel.HYPERPLANE, 1)
+    assert np.array_equal(a.get_travel_time_field_f32(),
+                          b.get_travel_time_field_f32())
+
+
+def test_batch_polish_matches_single(ekx, mixed_case):
+    """A batch of sources gives, per source, the field of the single-source
+    solve with the same options."""
+    w, slow = mixed_case
+    g = ekx.Geometry3D(w["nx"], w["ny"], w["nz"], w["h"])
+    srcs = [w["source"], (100.0, 1200.0, 0.0), (2000.0, 300.0, 1500.0)]
+    opts = ekx.SolverOptions(number_of_sweeps=2, factored_radius=w["radius"],
+                             precision=ekx.Precision.FP32_POLISH)
+    b = ekx.Batch3D()
+    b.initialize(opts, [(g, slow, s) for s in srcs])
+    b.solve()
+    for i, src in enumerate(srcs):
+        s = ekx.Solver3D()
+        s.initialize(opts, g)
+        s.set_slowness(slow)
+        s.set_source(src)
+        s.solve()
+        assert np.array_equal(b.get_travel_time_field(i),
+                              s.get_travel_time_field())
