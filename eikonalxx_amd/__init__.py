@@ -33,9 +33,12 @@ class SolverAlgorithm(enum.IntEnum):
 
 
 class Precision(enum.IntEnum):
-    FP64 = 0
-    FP32_POLISH = 1
-    FP32 = 2
+    """Storage and arithmetic of the sweeps (ekx_precision)."""
+    FP64 = 0            # double everywhere: the exact restatement
+    FP32_POLISH = 1     # FP32 sweeps, final iteration in FP64
+    FP32 = 2            # FP32 sweeps (FP64 inside the factored box)
+    FIXED32 = 3         # 32-bit scaled-integer storage, FP64 arithmetic
+    FIXED32_POLISH = 4  # FIXED32, final iteration in FP64
 
 
 class Schedule(enum.IntEnum):
@@ -408,6 +411,16 @@ class Solver3D:
         _check(self._L.ekx_solver3d_get_travel_times_f32(
             self._h, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
+
+    def get_travel_time_field_device(self):
+        """(device address, element size in bytes) of the field, borrowed
+        until the next solve (Solver3D::getTravelTimeFieldPointer,
+        src/solver3d.cpp:516-525)."""
+        ptr = ctypes.c_void_p()
+        size = ctypes.c_int()
+        _check(self._L.ekx_solver3d_get_travel_times_device(
+            self._h, ctypes.byref(ptr), ctypes.byref(size)))
+        return ptr.value, size.value
 
     def get_stats(self):
         st = SolveStats()

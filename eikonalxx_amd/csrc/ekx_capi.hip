@@ -3,6 +3,7 @@
 // of a solve and replays it from a captured hipGraph.
 #include "eikonalxx_amd.h"
 #include "ekx_kernels.hpp"
+#include "ekx_storage.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -18,6 +19,10 @@ using namespace ekx;
 
 namespace
 {
+
+// Blocks of the model-maximum reduction (FIXED32 modes).
+constexpr size_t kMaxPartials = 1024;
+
 
 thread_local std::string g_error;
 
@@ -135,7 +140,8 @@ ekx_status check_options(const ekx_solver_options *o)
     {
         return fail(EKX_EINVAL, "number_of_sweeps must be in [0, 65535]");
     }
-    if (o->precision < EKX_PRECISION_FP64 || o->precision > EKX_PRECISION_FP32)
+    if (o->precision < EKX_PRECISION_FP64 ||
+        o->precision > EKX_PRECISION_FIXED32_POLISH)
     {
         return fail(EKX_EINVAL, "unknown precision");
     }
@@ -270,10 +276,16 @@ struct Engine3D
     std::vector<int> modelOf;        // problem -> model
     std::vector<int64_t> modelOffset;
     int64_t totalNodes = 0, totalCells = 0, maxNodes = 0;
+    std::vector<int64_t> modelCells;
     DeviceBuffer<double> S64;
     DeviceBuffer<float> S32;
     DeviceBuffer<double> T64;
     DeviceBuffer<float> T32;
+    // scaled-integer storage (FIXED32 modes)
+    DeviceBuffer<uint32_t> SQ;
+    DeviceBuffer<uint32_t> TQ;
+    DeviceBuffer<double> partial;
+    std::vector<double> modelSMax;   // largest slowness of each model
     DeviceBuffer<Problem3D> dProblems;
     DeviceBuffer<ColumnItem> dItems;
     std::vector<int> diagStart;
@@ -334,9 +346,15 @@ struct Engine3D
     {
         return opt.precision == EKX_PRECISION_FP64;
     }
+    bool fixed32() const
+    {
+        return opt.precision == EKX_PRECISION_FIXED32 ||
+               opt.precision == EKX_PRECISION_FIXED32_POLISH;
+    }
     bool polish() const
     {
-        return opt.precision == EKX_PRECISION_FP32_POLISH &&
+        return (opt.precision == EKX_PRECISION_FP32_POLISH ||
+                opt.precision == EKX_PRECISION_FIXED32_POLISH) &&
                opt.schedule == EKX_SCHEDULE_COMPLETED &&
                opt.number_of_sweeps >= 1;
     }
@@ -377,6 +395,8 @@ struct Engine3D
         }
         // model offsets (cells)
         modelOffset.assign(nModels, -1);
+        modelCells.assign(nModels, 0);
+        modelSMax.assign(nModels, 0.0);
         totalCells = 0;
         for (int i = 0; i < n; ++i)
         {
@@ -385,7 +405,8 @@ struct Engine3D
             {
                 modelOffset[m] = totalCells;
                 const ekx_geometry3d &g = geoms[i];
-                totalCells += (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+                modelCells[m] = (int64_t) (g.nx - 1)*(g.ny - 1)*(g.nz - 1);
+                totalCells += modelCells[m];
             }
         }
         totalNodes = 0;
@@ -410,8 +431,22 @@ struct Engine3D
             maxNodes = std::max(maxNodes, nodes);
         }
         EKX_CHECK(S64.allocate((size_t) totalCells));
-        if (!sweeps_f64()) { EKX_CHECK(S32.allocate((size_t) totalCells)); }
+        const bool fp32 = !sweeps_f64() && !fixed32();
+        if (fp32) { EKX_CHECK(S32.allocate((size_t) totalCells)); }
         else { S32.release(); }
+        if (fixed32())
+        {
+            EKX_CHECK(SQ.allocate((size_t) totalCells));
+            EKX_CHECK(partial.allocate(kMaxPartials));
+            EKX_CHECK(TQ.allocate((size_t) totalNodes));
+            EKX_HIP(hipMemsetAsync(TQ.ptr, 0, TQ.count*sizeof(uint32_t),
+                                   stream));
+        }
+        else
+        {
+            SQ.release();
+            TQ.release();
+        }
         if (sweeps_f64() || polish())
         {
             EKX_CHECK(T64.allocate((size_t) totalNodes));
@@ -419,7 +454,7 @@ struct Engine3D
                                    stream));
         }
         else { T64.release(); }
-        if (!sweeps_f64())
+        if (fp32)
         {
             EKX_CHECK(T32.allocate((size_t) totalNodes));
             EKX_HIP(hipMemsetAsync(T32.ptr, 0, T32.count*sizeof(float),
@@ -492,6 +527,34 @@ struct Engine3D
         {
             EKX_HIP(launch_convert(totalCells, S64.ptr, S32.ptr, stream));
         }
+        if (SQ.ptr)
+        {
+            // Per model: the largest slowness fixes the code step
+            // (s = code * sMax / (2^32 - 2)), then the model is encoded.
+            std::vector<double> host(kMaxPartials);
+            for (size_t m = 0; m < modelCells.size(); ++m)
+            {
+                const double *sm = S64.ptr + modelOffset[m];
+                EKX_HIP(launch_block_max(modelCells[m], sm, partial.ptr,
+                                         (int) kMaxPartials, stream));
+                EKX_HIP(hipMemcpyAsync(host.data(), partial.ptr,
+                                       kMaxPartials*sizeof(double),
+                                       hipMemcpyDeviceToHost, stream));
+                EKX_HIP(hipStreamSynchronize(stream));
+                double sMax = 0.0;
+                for (double v : host) { sMax = std::max(sMax, v); }
+                if (!(sMax > 0.0) || !std::isfinite(sMax))
+                {
+                    return fail(EKX_EINVAL, "slowness model must be positive "
+                                            "and finite");
+                }
+                modelSMax[m] = sMax;
+                double step = sMax/(double) kTimeMaxCode;
+                EKX_HIP(launch_encode_slowness(modelCells[m], sm, 1.0/step,
+                                               SQ.ptr + modelOffset[m],
+                                               stream));
+            }
+        }
         EKX_HIP(hipStreamSynchronize(stream));
         haveModel = true;
         haveField = false;
@@ -525,11 +588,38 @@ struct Engine3D
             p.srcOffY = s.offset[1];
             p.srcOffZ = s.offset[2];
             p.srcSlowness = sSrc[i];
+            if (SQ.ptr) { set_fixed32_steps(i); }
         }
         EKX_HIP(hipMemcpyAsync(dProblems.ptr, problems.data(),
                                problems.size()*sizeof(Problem3D),
                                hipMemcpyHostToDevice, stream));
         return EKX_OK;
+    }
+
+    // Code steps of problem i in the FIXED32 modes.  Every finite travel
+    // time of the scheme is at most sMax times the largest Manhattan
+    // distance from the source to a node, plus a few cells: the local
+    // update never exceeds its smallest face neighbour + h * sMax
+    // (the 1D operators of finiteDifference), the initialisation sweeps
+    // reach every node through such a chain from the source cell, whose
+    // corners hold sSrc * |x - xs|, and refinement sweeps only lower
+    // values.  The slack of 8 cells also absorbs the half-step rounding of
+    // the stored codes along a chain.
+    void set_fixed32_steps(int i)
+    {
+        Problem3D &p = problems[i];
+        const ekx_geometry3d &g = geometry[i];
+        const double sMax = modelSMax[modelOf[i]];
+        const double lx = (g.nx - 1)*g.dx, ly = (g.ny - 1)*g.dy;
+        const double lz = (g.nz - 1)*g.dz;
+        const double mx = std::max(p.srcOffX, lx - p.srcOffX);
+        const double my = std::max(p.srcOffY, ly - p.srcOffY);
+        const double mz = std::max(p.srcOffZ, lz - p.srcOffZ);
+        const double hMax = std::max(g.dx, std::max(g.dy, g.dz));
+        const double tBound = sMax*(mx + my + mz + 8*hMax);
+        p.tStep = tBound/(double) kTimeMaxCode;
+        p.tScale = 1.0/p.tStep;
+        p.sStep = sMax/(double) kTimeMaxCode;
     }
 
     template <typename TS>
@@ -723,6 +813,26 @@ struct Engine3D
                                           st));
             EKX_HIP(hipEventRecord(evSeg[1], stream));
         }
+        else if (fixed32())
+        {
+            int itQ = polish() ? nIt - 1 : nIt;
+            EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, maxNodes,
+                                            TQ.ptr, stream));
+            EKX_HIP(hipEventRecord(evSeg[0], stream));
+            EKX_CHECK(run_segment<uint32_t>(0, TQ.ptr, SQ.ptr, true, itQ,
+                                            st));
+            EKX_HIP(hipEventRecord(evSeg[1], stream));
+            if (polish())
+            {
+                EKX_HIP(launch_decode_times(dProblems.ptr, nProb, maxNodes,
+                                            TQ.ptr, T64.ptr, stream));
+                EKX_HIP(hipEventRecord(evSeg[2], stream));
+                EKX_CHECK(run_segment<double>(1, T64.ptr, S64.ptr, false, 1,
+                                              st));
+                EKX_HIP(hipEventRecord(evSeg[3], stream));
+                polished = true;
+            }
+        }
         else
         {
             int itF32 = polish() ? nIt - 1 : nIt;
@@ -777,6 +887,20 @@ struct Engine3D
         {
             EKX_HIP(hipMemcpy(out, T64.ptr + off, (size_t) n*sizeof(double),
                               hipMemcpyDeviceToHost));
+        }
+        else if (TQ.ptr)
+        {
+            // the decoding of Store<uint32_t>::time
+            std::vector<uint32_t> tmp((size_t) n);
+            EKX_HIP(hipMemcpy(tmp.data(), TQ.ptr + off,
+                              (size_t) n*sizeof(uint32_t),
+                              hipMemcpyDeviceToHost));
+            const double step = problems[problem].tStep;
+            for (int64_t i = 0; i < n; ++i)
+            {
+                uint32_t c = tmp[(size_t) i];
+                out[i] = c == kTimeHugeCode ? kHuge : (double) c*step;
+            }
         }
         else
         {
@@ -1076,6 +1200,12 @@ ekx_status ekx_solver3d_get_travel_times_device(const ekx_solver3d *h,
         return fail(EKX_ESTATE, "Travel time field not yet computed");
     }
     const Engine3D &e = h->engine;
+    if (!e.fieldIsF64 && !e.T32.ptr)
+    {
+        return fail(EKX_ESTATE, "a FIXED32 field is stored as scaled "
+                                "integers; copy it with "
+                                "ekx_solver3d_get_travel_times");
+    }
     if (ptr)
     {
         *ptr = e.fieldIsF64 ? (const void *) e.T64.ptr

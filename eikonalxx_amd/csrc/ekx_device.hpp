@@ -44,6 +44,9 @@ struct Problem3D
     double srcSlowness;         // slowness of the source cell
     int64_t tOffset;            // first node of this problem's field
     int64_t sOffset;            // first cell of this problem's model
+    // Scaled-integer storage (uint32 codes): travel time = code * tStep,
+    // slowness = code * sStep; tScale = 1 / tStep.  Unused otherwise.
+    double tStep, tScale, sStep;
     NonUniform3D nu;
 };
 

@@ -6,9 +6,9 @@ namespace ekx
 {
 
 // Waves per workgroup of the column kernel.  Each wave owns 17 LDS planes
-// of 2 x 84 values (11.4 KB in FP32, 22.8 KB in FP64), so LDS (160 KB per
-// CU) bounds residency at 14 FP32 or 7 FP64 waves per CU; small blocks let
-// the dispatcher reach that bound.
+// of 2 x 84 values (11.4 KB for 4-byte elements, 22.8 KB for FP64), so LDS
+// (160 KB per CU) bounds residency at 14 or 7 waves per CU; small blocks
+// let the dispatcher reach that bound.
 template <typename TS> constexpr int column_waves()
 {
     return sizeof(TS) == 4 ? 2 : 1;
@@ -44,6 +44,16 @@ hipError_t launch_cell_velocity_to_slowness3d(int nx, int ny, int nz,
                                               const double *v, int ordering,
                                               double *slow,
                                               hipStream_t stream);
+// Scaled-integer storage: block maxima of a model (nPartial blocks), the
+// model's slowness codes round(s * scale), and every problem's travel-time
+// codes decoded to FP64.
+hipError_t launch_block_max(int64_t n, const double *v, double *partial,
+                            int nPartial, hipStream_t stream);
+hipError_t launch_encode_slowness(int64_t n, const double *in, double scale,
+                                  uint32_t *out, hipStream_t stream);
+hipError_t launch_decode_times(const Problem3D *dProblems, int nProblems,
+                               int64_t maxNodes, const uint32_t *in,
+                               double *out, hipStream_t stream);
 hipError_t launch_convert(int64_t n, const double *in, float *out,
                           hipStream_t stream);
 hipError_t launch_convert(int64_t n, const float *in, double *out,

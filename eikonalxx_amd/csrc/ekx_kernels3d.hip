@@ -9,6 +9,7 @@
 // schedules.
 #include "ekx_kernels.hpp"
 #include "ekx_local3d.hpp"
+#include "ekx_storage.hpp"
 
 namespace ekx
 {
@@ -85,13 +86,15 @@ init_travel_times3d(const Problem3D *__restrict__ problems,
         int iy = rem/p.nx;
         int ix = rem - iy*p.nx;
         int ox = ix - p.srcCellX, oy = iy - p.srcCellY, oz = iz - p.srcCellZ;
-        double v = kHuge;
+        TS v = Store<TS>::kHugeCode;
         if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1 && oz >= 0 && oz <= 1)
         {
-            v = analytic_time3d(ix, iy, iz, p.dx, p.dy, p.dz, p.srcOffX,
-                                p.srcOffY, p.srcOffZ, p.srcSlowness);
+            v = Store<TS>::code(
+                (typename Store<TS>::R) analytic_time3d(
+                    ix, iy, iz, p.dx, p.dy, p.dz, p.srcOffX, p.srcOffY,
+                    p.srcOffZ, p.srcSlowness), p);
         }
-        Tp[i] = (TS) v;
+        Tp[i] = v;
     }
 }
 
@@ -134,36 +137,39 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
     const int cxH = cellx(jx - 1), cxO = cellx(min(jx, ncx - 1));
     const int cyH = celly(jy - 1), cyO = celly(min(jy, ncy - 1));
     const int czH = cellz(jz - 1), czO = cellz(min(jz, ncz - 1));
+    using St = Store<TS>;
+    using R = typename St::R;
     const TS *Sp = S + p.sOffset;
     auto sAt = [&](int cx, int cy, int cz)
     {
-        return Sp[((int64_t) cz*ncy + cy)*ncx + cx];
+        return St::slowness(Sp[((int64_t) cz*ncy + cy)*ncx + cx], p);
     };
     TS *Tp = T + p.tOffset;
     auto tIdx = [&](int x, int y, int z)
     {
         return ((int64_t) z*ny + y)*nx + x;
     };
-    const TS s0 = sAt(cxH, cyH, czH);
-    const TS s1 = sAt(cxO, cyH, czH);
-    const TS s2 = sAt(cxO, cyO, czH);
-    const TS s3 = sAt(cxH, cyO, czH);
-    const TS s4 = sAt(cxH, cyH, czO);
-    const TS s5 = sAt(cxO, cyH, czO);
-    const TS s7 = sAt(cxH, cyO, czO);
+    const R s0 = sAt(cxH, cyH, czH);
+    const R s1 = sAt(cxO, cyH, czH);
+    const R s2 = sAt(cxO, cyO, czH);
+    const R s3 = sAt(cxH, cyO, czH);
+    const R s4 = sAt(cxH, cyH, czO);
+    const R s5 = sAt(cxO, cyH, czO);
+    const R s7 = sAt(cxH, cyO, czO);
     const int64_t i0 = tIdx(ix, iy, iz);
-    const TS t0 = Tp[i0];
-    const TS t1 = Tp[tIdx(ixU, iy, iz)];
-    const TS t2 = Tp[tIdx(ixU, iyU, iz)];
-    const TS t3 = Tp[tIdx(ix, iyU, iz)];
-    const TS t4 = Tp[tIdx(ix, iy, izU)];
-    const TS t5 = Tp[tIdx(ixU, iy, izU)];
-    const TS t6 = Tp[tIdx(ixU, iyU, izU)];
-    const TS t7 = Tp[tIdx(ix, iyU, izU)];
+    const TS own = Tp[i0];
+    const R t1 = St::time(Tp[tIdx(ixU, iy, iz)], p);
+    const R t2 = St::time(Tp[tIdx(ixU, iyU, iz)], p);
+    const R t3 = St::time(Tp[tIdx(ix, iyU, iz)], p);
+    const R t4 = St::time(Tp[tIdx(ix, iy, izU)], p);
+    const R t5 = St::time(Tp[tIdx(ixU, iy, izU)], p);
+    const R t6 = St::time(Tp[tIdx(ixU, iyU, izU)], p);
+    const R t7 = St::time(Tp[tIdx(ix, iyU, izU)], p);
     const Sweep3D sw = make_sweep3d(sweep);
-    const TS tUpd = local_update3d<TS>(p, sw, ix, iy, iz, s0, s1, s2, s3, s4,
-                                       s5, s7, t1, t2, t3, t4, t5, t6, t7);
-    if (tUpd < t0) { Tp[i0] = tUpd; }
+    const R tUpd = local_update3d<R>(p, sw, ix, iy, iz, s0, s1, s2, s3, s4,
+                                     s5, s7, t1, t2, t3, t4, t5, t6, t7);
+    const TS c = St::code(tUpd, p);
+    if (c < own) { Tp[i0] = c; }
 }
 
 // ---------------------------------------------------------------------------
@@ -306,7 +312,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         setup(lane, tXY0, sXY0, tOk0);
         setup(has1 ? lane + 64 : lane, tXY1, sXY1, tOk1);
     }
-    const TS huge = (TS) kHuge;
+    using St = Store<TS>;
+    using R = typename St::R;
+    const TS hugeCode = St::kHugeCode;
     auto loadPlane = [&](int q, TS &vT0, TS &vT1, TS &vS0, TS &vS1)
     {
         const int jz = P0 + q;
@@ -331,8 +339,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         loadPlane(min(u, nPlanes - 1), bT0[u], bT1[u], bS0[u], bS1[u]);
         __builtin_amdgcn_sched_barrier(0);
     }
-    TS pt4 = huge, pt5 = huge, pt6 = huge, pt7 = huge;
-    TS ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
+    // z-neighbours (plane q - 1) of the lane's node, decoded
+    R pt4 = (R) kHuge, pt5 = (R) kHuge, pt6 = (R) kHuge, pt7 = (R) kHuge;
+    R ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
     // The node is local (a+1, b+1).  Its neighbours (x-1, y-1, xy-1) and
     // its 4 cells of one plane share these four LDS offsets.
     const int oA = lidx(a, b), oB = lidx(a + 1, b), oC = lidx(a, b + 1);
@@ -363,9 +372,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                                               : kColumnRing;
                 TS *dT = myRing[slot][0];
                 TS *dS = myRing[slot][1];
-                dT[dst0] = tOk0 ? bT0[u] : huge;
+                dT[dst0] = tOk0 ? bT0[u] : hugeCode;
                 dS[dst0] = bS0[u];
-                dT[dst1] = tOk1 ? bT1[u] : huge;
+                dT[dst1] = tOk1 ? bT1[u] : hugeCode;
                 dS[dst1] = bS1[u];
             }
             // (b) prefetch plane tc + DL into the freed registers (past the
@@ -382,41 +391,43 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             {
                 TS *sT = myRing[q & (kColumnRing - 1)][0];
                 const TS *sS = myRing[q & (kColumnRing - 1)][1];
-                TS tOwn = sT[oD];
-                const TS t1 = sT[oC];
-                const TS t2 = sT[oA];
-                const TS t3 = sT[oB];
-                const TS c00 = sS[oA];
-                const TS c10 = sS[oB];
-                const TS c01 = sS[oC];
-                const TS c11 = sS[oD];
+                const TS ownCode = sT[oD];
+                R tOwn = St::time(ownCode, p);
+                const R t1 = St::time(sT[oC], p);
+                const R t2 = St::time(sT[oA], p);
+                const R t3 = St::time(sT[oB], p);
+                const R c00 = St::slowness(sS[oA], p);
+                const R c10 = St::slowness(sS[oB], p);
+                const R c01 = St::slowness(sS[oC], p);
+                const R c11 = St::slowness(sS[oD], p);
                 if (q > 0 && upd)
                 {
                     const int jz = P0 + q;
                     const int iz = negZ ? nz - 1 - jz : jz;
-                    TS tUpd;
+                    R tUpd;
                     if (FACTORED)
                     {
-                        tUpd = local_update3d<TS>(
+                        tUpd = local_update3d<R>(
                             p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00,
                             c10, c01, t1, t2, t3, pt4, pt5, pt6, pt7);
                     }
                     else if (p.uniform)
                     {
-                        tUpd = cartesian_uniform3d<TS>(
-                            (TS) p.dx, ps00, ps10, ps11, ps01, c00, c10, c01,
+                        tUpd = cartesian_uniform3d<R>(
+                            (R) p.dx, ps00, ps10, ps11, ps01, c00, c10, c01,
                             t1, t2, t3, pt4, pt5, pt6, pt7);
                     }
                     else
                     {
-                        tUpd = cartesian_nonuniform3d<TS>(
+                        tUpd = cartesian_nonuniform3d<R>(
                             p, ps00, ps10, ps11, ps01, c00, c10, c01, t1, t2,
                             t3, pt4, pt5, pt6, pt7);
                     }
-                    if (tUpd < tOwn)
+                    const TS c = St::code(tUpd, p);
+                    if (c < ownCode)
                     {
-                        tOwn = tUpd;
-                        sT[oD] = tUpd;
+                        tOwn = St::finite_time(c, p);
+                        sT[oD] = c;
                     }
                 }
                 pt4 = tOwn;
@@ -528,6 +539,62 @@ convert_kernel(int64_t n, const TSrc *__restrict__ in, TD *__restrict__ out)
 }
 
 // ---------------------------------------------------------------------------
+// Scaled-integer storage: model maximum, slowness codes, decoded fields.
+// ---------------------------------------------------------------------------
+// Block maxima of v[0..n) into partial[blockIdx.x] (the host reduces them).
+__global__ void __launch_bounds__(256)
+block_max_kernel(int64_t n, const double *__restrict__ v,
+                 double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        m = fmax(m, v[i]);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1)
+    {
+        if ((int) threadIdx.x < k)
+        {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + k]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[blockIdx.x] = red[0]; }
+}
+
+__global__ void __launch_bounds__(256)
+encode_slowness_kernel(int64_t n, const double *__restrict__ in,
+                       double scale, uint32_t *__restrict__ out)
+{
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        double q = rint(in[i]*scale);
+        out[i] = (uint32_t) fmin(fmax(q, 0.0), (double) kTimeMaxCode);
+    }
+}
+
+// Travel-time codes of every problem to FP64 (blockIdx.y is the problem).
+__global__ void __launch_bounds__(256)
+decode_times_kernel(const Problem3D *__restrict__ problems,
+                    const uint32_t *__restrict__ in, double *__restrict__ out)
+{
+    const Problem3D &p = problems[blockIdx.y];
+    const int64_t n = (int64_t) p.nx*p.ny*p.nz;
+    const uint32_t *src = in + p.tOffset;
+    double *dst = out + p.tOffset;
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        dst[i] = Store<uint32_t>::time(src[i], p);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(int64_t n, int block, unsigned cap)
@@ -631,6 +698,32 @@ hipError_t launch_convert(int64_t n, const float *in, double *out,
     return hipGetLastError();
 }
 
+hipError_t launch_block_max(int64_t n, const double *v, double *partial,
+                            int nPartial, hipStream_t stream)
+{
+    hipLaunchKernelGGL(block_max_kernel, dim3((unsigned) nPartial), dim3(256),
+                       0, stream, n, v, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode_slowness(int64_t n, const double *in, double scale,
+                                  uint32_t *out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(encode_slowness_kernel, dim3(grid_for(n, 256, 16384)),
+                       dim3(256), 0, stream, n, in, scale, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_times(const Problem3D *dProblems, int nProblems,
+                               int64_t maxNodes, const uint32_t *in,
+                               double *out, hipStream_t stream)
+{
+    dim3 grid(grid_for(maxNodes, 256, 4096), (unsigned) nProblems);
+    hipLaunchKernelGGL(decode_times_kernel, grid, dim3(256), 0, stream,
+                       dProblems, in, out);
+    return hipGetLastError();
+}
+
 template hipError_t launch_init3d<float>(const Problem3D *, int, int64_t,
                                          float *, hipStream_t);
 template hipError_t launch_init3d<double>(const Problem3D *, int, int64_t,
@@ -658,5 +751,21 @@ template hipError_t launch_factored_columns3d<double>(const Problem3D *, int,
                                                       double *,
                                                       const double *,
                                                       hipStream_t);
+template hipError_t launch_init3d<uint32_t>(const Problem3D *, int, int64_t,
+                                            uint32_t *, hipStream_t);
+template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
+                                                  int, int, int, int, int,
+                                                  uint32_t *,
+                                                  const uint32_t *,
+                                                  hipStream_t);
+template hipError_t launch_columns3d<uint32_t>(const Problem3D *,
+                                               const ColumnItem *, int, int,
+                                               int, int, int, uint32_t *,
+                                               const uint32_t *, hipStream_t);
+template hipError_t launch_factored_columns3d<uint32_t>(const Problem3D *,
+                                                        int, int, int, int,
+                                                        int, uint32_t *,
+                                                        const uint32_t *,
+                                                        hipStream_t);
 
 } // namespace ekx

@@ -57,7 +57,12 @@ typedef enum
     EKX_PRECISION_FP64 = 0,        /* every sweep in double                */
     EKX_PRECISION_FP32_POLISH = 1, /* FP32 sweeps, FP64 near the source,
                                       final iteration (8 sweeps) in FP64   */
-    EKX_PRECISION_FP32 = 2         /* FP32 sweeps, FP64 near the source    */
+    EKX_PRECISION_FP32 = 2,        /* FP32 sweeps, FP64 near the source    */
+    EKX_PRECISION_FIXED32 = 3,     /* travel times and slowness stored as
+                                      32-bit scaled integers (step ~1e-9 of
+                                      the problem's travel-time bound),
+                                      FP64 arithmetic                      */
+    EKX_PRECISION_FIXED32_POLISH = 4 /* FIXED32, final iteration in FP64   */
 } ekx_precision;
 
 /* Sweep schedule of Solver3D::solve.  COMPLETED: 8 initialisation sweeps
@@ -130,7 +135,8 @@ typedef struct
     int32_t iterations_run;       /* refinement iterations actually run    */
     int32_t reserved;
     double last_solve_ms;         /* device time of the whole last solve   */
-    double sweep_ms_fp32;         /* device time of the FP32 sweep launches */
+    double sweep_ms_fp32;         /* device time of the sweep launches on
+                                     4-byte storage (FP32 or FIXED32)      */
     double sweep_ms_fp64;         /* device time of the FP64 sweep launches */
 } ekx_solve_stats;
 
@@ -206,7 +212,9 @@ EKX_API ekx_status ekx_solver3d_get_travel_times(const ekx_solver3d *solver,
 EKX_API ekx_status ekx_solver3d_get_travel_times_f32(
     const ekx_solver3d *solver, int64_t n, float *out);
 /* Device pointer of the travel-time field (borrowed until the next solve or
- * initialize), and its element size (4 or 8).  Corresponds to
+ * initialize), and its element size (4: float, 8: double).  A FIXED32 field
+ * without polish is stored as scaled integers and is not exposed here
+ * (EKX_ESTATE); read it with ekx_solver3d_get_travel_times.  Corresponds to
  * Solver3D::getTravelTimeFieldPointer (src/solver3d.cpp:516-525). */
 EKX_API ekx_status ekx_solver3d_get_travel_times_device(
     const ekx_solver3d *solver, const void **device_ptr,

@@ -1,0 +1,1009 @@
+// eikonalxx_amd.hpp -- C++ front end of the MI355X fast-sweeping solver.
+//
+// Header-only.  The classes keep the names, methods, argument meanings and
+// exceptions of the reference's public C++ API (include/eikonalxx/*.hpp of
+// uofuseismo/eikonalxx) so that code written against
+//     EikonalXX::Solver3D<double> / Solver2D<double>
+// compiles against this header unchanged; every solve runs on the GPU
+// through the C ABI of eikonalxx_amd.h.  Link with -leikonalxx_amd.
+//
+// Errors: the C ABI's EKX_EINVAL becomes std::invalid_argument, every other
+// failure std::runtime_error, as the reference throws them.
+//
+// The classes live in namespace eikonalxx_amd; the alias EikonalXX is
+// provided unless EKX_NO_EIKONALXX_ALIAS is defined.
+#pragma once
+#include "eikonalxx_amd.h"
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+namespace eikonalxx_amd
+{
+
+// include/eikonalxx/enums.hpp:7-89
+enum class Ordering2D { Natural = 0, ZX = 0, XZ = 1 };
+enum class Ordering3D { Natural = 0, ZYX = 0, XYZ = 1 };
+enum class SolverAlgorithm { LevelSetMethod = 0, FastSweepingMethod = 1 };
+enum class Verbosity { Error = 0, Warning = 1, Info = 2, Debug = 3 };
+// Extensions of this implementation (no reference counterpart).
+enum class Precision
+{
+    FP64 = EKX_PRECISION_FP64,
+    FP32Polish = EKX_PRECISION_FP32_POLISH,
+    FP32 = EKX_PRECISION_FP32,
+    Fixed32 = EKX_PRECISION_FIXED32,
+    Fixed32Polish = EKX_PRECISION_FIXED32_POLISH
+};
+enum class Schedule
+{
+    Completed = EKX_SCHEDULE_COMPLETED,
+    Shipped = EKX_SCHEDULE_SHIPPED
+};
+
+namespace detail
+{
+inline void check(ekx_status status)
+{
+    if (status == EKX_OK) { return; }
+    std::string msg = ekx_last_error();
+    if (status == EKX_EINVAL) { throw std::invalid_argument(msg); }
+    throw std::runtime_error(msg);
+}
+inline void require(bool ok, const char *what)
+{
+    if (!ok) { throw std::runtime_error(what); }
+}
+} // namespace detail
+
+// ---------------------------------------------------------------------------
+// SolverOptions (include/eikonalxx/solverOptions.hpp:14-110)
+// ---------------------------------------------------------------------------
+class SolverOptions
+{
+public:
+    SolverOptions() { ekx_solver_options_default(&mOptions); }
+    void setNumberOfSweeps(uint16_t nSweeps) noexcept
+    {
+        mOptions.number_of_sweeps = nSweeps;
+    }
+    [[nodiscard]] int getNumberOfSweeps() const noexcept
+    {
+        return mOptions.number_of_sweeps;
+    }
+    void setConvergenceTolerance(double tolerance) noexcept
+    {
+        mOptions.convergence_tolerance = tolerance;
+    }
+    [[nodiscard]] double getConvergenceTolerance() const noexcept
+    {
+        return mOptions.convergence_tolerance;
+    }
+    void setFactoredEikonalEquationSolverRadius(int epsilon) noexcept
+    {
+        mOptions.factored_radius = epsilon;
+    }
+    [[nodiscard]] int getFactoredEikonalEquationSolverRadius() const noexcept
+    {
+        return mOptions.factored_radius;
+    }
+    void setVerbosity(Verbosity verbosity) noexcept
+    {
+        mOptions.verbosity = static_cast<int>(verbosity);
+    }
+    [[nodiscard]] Verbosity getVerbosity() const noexcept
+    {
+        return static_cast<Verbosity>(mOptions.verbosity);
+    }
+    void setAlgorithm(SolverAlgorithm algorithm) noexcept
+    {
+        mOptions.algorithm = static_cast<int>(algorithm);
+    }
+    [[nodiscard]] SolverAlgorithm getAlgorithm() const noexcept
+    {
+        return static_cast<SolverAlgorithm>(mOptions.algorithm);
+    }
+    void clear() noexcept { ekx_solver_options_default(&mOptions); }
+    // Extensions: storage/arithmetic of the sweeps and the schedule.
+    void setPrecision(Precision precision) noexcept
+    {
+        mOptions.precision = static_cast<int>(precision);
+    }
+    [[nodiscard]] Precision getPrecision() const noexcept
+    {
+        return static_cast<Precision>(mOptions.precision);
+    }
+    void setSchedule(Schedule schedule) noexcept
+    {
+        mOptions.schedule = static_cast<int>(schedule);
+    }
+    [[nodiscard]] Schedule getSchedule() const noexcept
+    {
+        return static_cast<Schedule>(mOptions.schedule);
+    }
+    [[nodiscard]] const ekx_solver_options &c() const noexcept
+    {
+        return mOptions;
+    }
+private:
+    ekx_solver_options mOptions{};
+};
+
+// ---------------------------------------------------------------------------
+// Geometry3D / Geometry2D (include/eikonalxx/geometry3d.hpp:12-170,
+// geometry2d.hpp; validation as src/geometry3d.cpp:73-261)
+// ---------------------------------------------------------------------------
+namespace detail
+{
+struct Axis
+{
+    int n = 0;
+    double d = 0;
+    double origin = 0;
+    bool haveN = false, haveD = false;
+    void setN(int v, const char *name)
+    {
+        if (v < 3)
+        {
+            throw std::invalid_argument(std::string("Number of grid points "
+                                                    "in ") + name + " = " +
+                                        std::to_string(v) +
+                                        " must be at least 3");
+        }
+        n = v;
+        haveN = true;
+    }
+    void setD(double v, const char *name)
+    {
+        if (!(v > 0))
+        {
+            throw std::invalid_argument(std::string("Grid spacing in ") +
+                                        name + " = " + std::to_string(v) +
+                                        " must be positive");
+        }
+        d = v;
+        haveD = true;
+    }
+    int getN(const char *name) const
+    {
+        if (!haveN)
+        {
+            throw std::runtime_error(std::string("Number of grid points in ")
+                                     + name + " not yet set");
+        }
+        return n;
+    }
+    double getD(const char *name) const
+    {
+        if (!haveD)
+        {
+            throw std::runtime_error(std::string("Grid spacing in ") + name +
+                                     " not yet set");
+        }
+        return d;
+    }
+};
+} // namespace detail
+
+class Geometry3D
+{
+public:
+    void setNumberOfGridPointsInX(int nx) { mX.setN(nx, "x"); }
+    void setNumberOfGridPointsInY(int ny) { mY.setN(ny, "y"); }
+    void setNumberOfGridPointsInZ(int nz) { mZ.setN(nz, "z"); }
+    [[nodiscard]] int getNumberOfGridPointsInX() const { return mX.getN("x"); }
+    [[nodiscard]] int getNumberOfGridPointsInY() const { return mY.getN("y"); }
+    [[nodiscard]] int getNumberOfGridPointsInZ() const { return mZ.getN("z"); }
+    [[nodiscard]] int getNumberOfCellsInX() const { return mX.getN("x") - 1; }
+    [[nodiscard]] int getNumberOfCellsInY() const { return mY.getN("y") - 1; }
+    [[nodiscard]] int getNumberOfCellsInZ() const { return mZ.getN("z") - 1; }
+    [[nodiscard]] bool haveNumberOfGridPointsInX() const noexcept
+    { return mX.haveN; }
+    [[nodiscard]] bool haveNumberOfGridPointsInY() const noexcept
+    { return mY.haveN; }
+    [[nodiscard]] bool haveNumberOfGridPointsInZ() const noexcept
+    { return mZ.haveN; }
+    void setGridSpacingInX(double dx) { mX.setD(dx, "x"); }
+    void setGridSpacingInY(double dy) { mY.setD(dy, "y"); }
+    void setGridSpacingInZ(double dz) { mZ.setD(dz, "z"); }
+    [[nodiscard]] double getGridSpacingInX() const { return mX.getD("x"); }
+    [[nodiscard]] double getGridSpacingInY() const { return mY.getD("y"); }
+    [[nodiscard]] double getGridSpacingInZ() const { return mZ.getD("z"); }
+    [[nodiscard]] bool haveGridSpacingInX() const noexcept { return mX.haveD; }
+    [[nodiscard]] bool haveGridSpacingInY() const noexcept { return mY.haveD; }
+    [[nodiscard]] bool haveGridSpacingInZ() const noexcept { return mZ.haveD; }
+    void setOriginInX(double x0) noexcept { mX.origin = x0; }
+    void setOriginInY(double y0) noexcept { mY.origin = y0; }
+    void setOriginInZ(double z0) noexcept { mZ.origin = z0; }
+    [[nodiscard]] double getOriginInX() const noexcept { return mX.origin; }
+    [[nodiscard]] double getOriginInY() const noexcept { return mY.origin; }
+    [[nodiscard]] double getOriginInZ() const noexcept { return mZ.origin; }
+    [[nodiscard]] int getNumberOfGridPoints() const
+    {
+        return getNumberOfGridPointsInX()*getNumberOfGridPointsInY()*
+               getNumberOfGridPointsInZ();
+    }
+    [[nodiscard]] int getNumberOfCells() const
+    {
+        return getNumberOfCellsInX()*getNumberOfCellsInY()*
+               getNumberOfCellsInZ();
+    }
+    void clear() noexcept { mX = {}; mY = {}; mZ = {}; }
+    // The C descriptor; src/solver3d.cpp:153-176 rejects unset fields.
+    [[nodiscard]] ekx_geometry3d c() const
+    {
+        const char *axes[] = {"x", "y", "z"};
+        const detail::Axis *a[] = {&mX, &mY, &mZ};
+        for (int k = 0; k < 3; ++k)
+        {
+            if (!a[k]->haveN)
+            {
+                throw std::invalid_argument(std::string("Grid points in ") +
+                                            axes[k] + " not set on geometry");
+            }
+            if (!a[k]->haveD)
+            {
+                throw std::invalid_argument(std::string("Grid spacing in ") +
+                                            axes[k] + " not set on geometry");
+            }
+        }
+        ekx_geometry3d g{};
+        g.nx = mX.n; g.ny = mY.n; g.nz = mZ.n;
+        g.dx = mX.d; g.dy = mY.d; g.dz = mZ.d;
+        g.x0 = mX.origin; g.y0 = mY.origin; g.z0 = mZ.origin;
+        return g;
+    }
+private:
+    detail::Axis mX, mY, mZ;
+};
+
+class Geometry2D
+{
+public:
+    void setNumberOfGridPointsInX(int nx) { mX.setN(nx, "x"); }
+    void setNumberOfGridPointsInZ(int nz) { mZ.setN(nz, "z"); }
+    [[nodiscard]] int getNumberOfGridPointsInX() const { return mX.getN("x"); }
+    [[nodiscard]] int getNumberOfGridPointsInZ() const { return mZ.getN("z"); }
+    [[nodiscard]] int getNumberOfCellsInX() const { return mX.getN("x") - 1; }
+    [[nodiscard]] int getNumberOfCellsInZ() const { return mZ.getN("z") - 1; }
+    [[nodiscard]] bool haveNumberOfGridPointsInX() const noexcept
+    { return mX.haveN; }
+    [[nodiscard]] bool haveNumberOfGridPointsInZ() const noexcept
+    { return mZ.haveN; }
+    void setGridSpacingInX(double dx) { mX.setD(dx, "x"); }
+    void setGridSpacingInZ(double dz) { mZ.setD(dz, "z"); }
+    [[nodiscard]] double getGridSpacingInX() const { return mX.getD("x"); }
+    [[nodiscard]] double getGridSpacingInZ() const { return mZ.getD("z"); }
+    [[nodiscard]] bool haveGridSpacingInX() const noexcept { return mX.haveD; }
+    [[nodiscard]] bool haveGridSpacingInZ() const noexcept { return mZ.haveD; }
+    void setOriginInX(double x0) noexcept { mX.origin = x0; }
+    void setOriginInZ(double z0) noexcept { mZ.origin = z0; }
+    [[nodiscard]] double getOriginInX() const noexcept { return mX.origin; }
+    [[nodiscard]] double getOriginInZ() const noexcept { return mZ.origin; }
+    [[nodiscard]] int getNumberOfGridPoints() const
+    {
+        return getNumberOfGridPointsInX()*getNumberOfGridPointsInZ();
+    }
+    [[nodiscard]] int getNumberOfCells() const
+    {
+        return getNumberOfCellsInX()*getNumberOfCellsInZ();
+    }
+    void clear() noexcept { mX = {}; mZ = {}; }
+    [[nodiscard]] ekx_geometry2d c() const
+    {
+        const char *axes[] = {"x", "z"};
+        const detail::Axis *a[] = {&mX, &mZ};
+        for (int k = 0; k < 2; ++k)
+        {
+            if (!a[k]->haveN)
+            {
+                throw std::invalid_argument(std::string("Grid points in ") +
+                                            axes[k] + " not set on geometry");
+            }
+            if (!a[k]->haveD)
+            {
+                throw std::invalid_argument(std::string("Grid spacing in ") +
+                                            axes[k] + " not set on geometry");
+            }
+        }
+        ekx_geometry2d g{};
+        g.nx = mX.n; g.nz = mZ.n;
+        g.dx = mX.d; g.dz = mZ.d;
+        g.x0 = mX.origin; g.z0 = mZ.origin;
+        return g;
+    }
+private:
+    detail::Axis mX, mZ;
+};
+
+// ---------------------------------------------------------------------------
+// Source3D / Source2D / Station2D / Station3D (include/eikonalxx/
+// source3d.hpp:14-140, src/source3d.cpp:161-330): offset = x - x0,
+// cell = min(nCell - 1, floor(offset / d)).
+// ---------------------------------------------------------------------------
+namespace detail
+{
+struct Location
+{
+    double location = 0, offset = 0;
+    int cell = 0;
+    bool have = false;
+    void set(double v, int n, double d, double origin, const char *name)
+    {
+        const double vmax = origin + (n - 1)*d;
+        if (v < origin || v > vmax)
+        {
+            throw std::invalid_argument(std::string(name) + " = " +
+                                        std::to_string(v) + " must be in [" +
+                                        std::to_string(origin) + "," +
+                                        std::to_string(vmax) + "]");
+        }
+        location = v;
+        offset = v - origin;
+        cell = std::min(n - 2, static_cast<int>(offset/d));
+        have = true;
+    }
+    const Location &get(const char *name) const
+    {
+        if (!have)
+        {
+            throw std::runtime_error(std::string(name) +
+                                     " location not yet set");
+        }
+        return *this;
+    }
+};
+} // namespace detail
+
+class Source3D
+{
+public:
+    void setGeometry(const Geometry3D &geometry)
+    {
+        (void) geometry.c();   // every field must be set
+        mGeometry = geometry;
+        mHaveGeometry = true;
+        mX = {}; mY = {}; mZ = {};
+    }
+    [[nodiscard]] Geometry3D getGeometry() const
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        return mGeometry;
+    }
+    [[nodiscard]] bool haveGeometry() const noexcept { return mHaveGeometry; }
+    void setLocationInX(double x)
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        mX.set(x, mGeometry.getNumberOfGridPointsInX(),
+               mGeometry.getGridSpacingInX(), mGeometry.getOriginInX(), "x");
+    }
+    void setLocationInY(double y)
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        mY.set(y, mGeometry.getNumberOfGridPointsInY(),
+               mGeometry.getGridSpacingInY(), mGeometry.getOriginInY(), "y");
+    }
+    void setLocationInZ(double z)
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        mZ.set(z, mGeometry.getNumberOfGridPointsInZ(),
+               mGeometry.getGridSpacingInZ(), mGeometry.getOriginInZ(), "z");
+    }
+    void setZToFreeSurface() { setLocationInZ(getGeometry().getOriginInZ()); }
+    [[nodiscard]] double getLocationInX() const
+    { return mX.get("x").location; }
+    [[nodiscard]] double getLocationInY() const
+    { return mY.get("y").location; }
+    [[nodiscard]] double getLocationInZ() const
+    { return mZ.get("z").location; }
+    [[nodiscard]] double getOffsetInX() const { return mX.get("x").offset; }
+    [[nodiscard]] double getOffsetInY() const { return mY.get("y").offset; }
+    [[nodiscard]] double getOffsetInZ() const { return mZ.get("z").offset; }
+    [[nodiscard]] int getCellInX() const { return mX.get("x").cell; }
+    [[nodiscard]] int getCellInY() const { return mY.get("y").cell; }
+    [[nodiscard]] int getCellInZ() const { return mZ.get("z").cell; }
+    [[nodiscard]] bool haveLocationInX() const noexcept { return mX.have; }
+    [[nodiscard]] bool haveLocationInY() const noexcept { return mY.have; }
+    [[nodiscard]] bool haveLocationInZ() const noexcept { return mZ.have; }
+    [[nodiscard]] int getCell() const
+    {
+        const int ncx = mGeometry.getNumberOfCellsInX();
+        const int ncy = mGeometry.getNumberOfCellsInY();
+        return (getCellInZ()*ncy + getCellInY())*ncx + getCellInX();
+    }
+    void clear() noexcept
+    {
+        mGeometry.clear();
+        mHaveGeometry = false;
+        mX = {}; mY = {}; mZ = {};
+    }
+private:
+    Geometry3D mGeometry;
+    bool mHaveGeometry = false;
+    detail::Location mX, mY, mZ;
+};
+
+class Source2D
+{
+public:
+    void setGeometry(const Geometry2D &geometry)
+    {
+        (void) geometry.c();
+        mGeometry = geometry;
+        mHaveGeometry = true;
+        mX = {}; mZ = {};
+    }
+    [[nodiscard]] Geometry2D getGeometry() const
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        return mGeometry;
+    }
+    [[nodiscard]] bool haveGeometry() const noexcept { return mHaveGeometry; }
+    void setLocationInX(double x)
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        mX.set(x, mGeometry.getNumberOfGridPointsInX(),
+               mGeometry.getGridSpacingInX(), mGeometry.getOriginInX(), "x");
+    }
+    void setLocationInZ(double z)
+    {
+        detail::require(mHaveGeometry, "Geometry not yet set");
+        mZ.set(z, mGeometry.getNumberOfGridPointsInZ(),
+               mGeometry.getGridSpacingInZ(), mGeometry.getOriginInZ(), "z");
+    }
+    void setZToFreeSurface() { setLocationInZ(getGeometry().getOriginInZ()); }
+    [[nodiscard]] double getLocationInX() const
+    { return mX.get("x").location; }
+    [[nodiscard]] double getLocationInZ() const
+    { return mZ.get("z").location; }
+    [[nodiscard]] double getOffsetInX() const { return mX.get("x").offset; }
+    [[nodiscard]] double getOffsetInZ() const { return mZ.get("z").offset; }
+    [[nodiscard]] int getCellInX() const { return mX.get("x").cell; }
+    [[nodiscard]] int getCellInZ() const { return mZ.get("z").cell; }
+    [[nodiscard]] bool haveLocationInX() const noexcept { return mX.have; }
+    [[nodiscard]] bool haveLocationInZ() const noexcept { return mZ.have; }
+    [[nodiscard]] int getCell() const
+    {
+        return getCellInZ()*mGeometry.getNumberOfCellsInX() + getCellInX();
+    }
+    void clear() noexcept
+    {
+        mGeometry.clear();
+        mHaveGeometry = false;
+        mX = {}; mZ = {};
+    }
+private:
+    Geometry2D mGeometry;
+    bool mHaveGeometry = false;
+    detail::Location mX, mZ;
+};
+
+// A receiver has the same location semantics as a source
+// (src/station2d.cpp, src/station3d.cpp).
+using Station2D = Source2D;
+using Station3D = Source3D;
+
+// ---------------------------------------------------------------------------
+// Model3D<T> / Model2D<T> (include/eikonalxx/model3d.hpp:18-150).  The
+// model keeps the caller's velocities; the cell slowness is formed on the
+// GPU when a solver takes the model (src/model3d.cpp:45-151).
+// ---------------------------------------------------------------------------
+template <class T>
+class Model3D
+{
+public:
+    void initialize(const Geometry3D &geometry)
+    {
+        (void) geometry.c();
+        mGeometry = geometry;
+        mInitialized = true;
+        mVelocity.clear();
+        mHave = false;
+    }
+    [[nodiscard]] bool isInitialized() const noexcept { return mInitialized; }
+    [[nodiscard]] int getNumberOfGridPoints() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry.getNumberOfGridPoints();
+    }
+    [[nodiscard]] int getNumberOfCells() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry.getNumberOfCells();
+    }
+    [[nodiscard]] Geometry3D getGeometry() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry;
+    }
+    void setNodalVelocities(int nGrid, const T velocity[],
+                            Ordering3D ordering)
+    {
+        set(nGrid, getNumberOfGridPoints(), velocity, ordering, true,
+            "Number of grid points");
+    }
+    template <typename U>
+    void setCellularVelocities(int nCell, const U velocity[],
+                               Ordering3D ordering)
+    {
+        set(nCell, getNumberOfCells(), velocity, ordering, false,
+            "Number of cells");
+    }
+    [[nodiscard]] bool haveVelocities() const noexcept { return mHave; }
+    // Slowness of one cell (src/model3d.cpp:436-466).
+    [[nodiscard]] T getSlowness(int iCellX, int iCellY, int iCellZ) const
+    {
+        detail::require(mHave, "Velocity model not yet set");
+        const int ncx = mGeometry.getNumberOfCellsInX();
+        const int ncy = mGeometry.getNumberOfCellsInY();
+        const int ncz = mGeometry.getNumberOfCellsInZ();
+        if (iCellX < 0 || iCellX >= ncx || iCellY < 0 || iCellY >= ncy ||
+            iCellZ < 0 || iCellZ >= ncz)
+        {
+            throw std::invalid_argument("cell index out of range");
+        }
+        const int nx = ncx + 1, ny = ncy + 1, nz = ncz + 1;
+        const bool natural = mOrdering == Ordering3D::Natural;
+        if (!mNodal)
+        {
+            size_t i = natural ? ((size_t) iCellZ*ncy + iCellY)*ncx + iCellX
+                               : ((size_t) iCellX*ncy + iCellY)*ncz + iCellZ;
+            return static_cast<T>(1.0/mVelocity[i]);
+        }
+        double sum = 0;
+        for (int k = 0; k < 2; ++k)
+        {
+            for (int j = 0; j < 2; ++j)
+            {
+                for (int i = 0; i < 2; ++i)
+                {
+                    int x = iCellX + i, y = iCellY + j, z = iCellZ + k;
+                    size_t idx = natural ? ((size_t) z*ny + y)*nx + x
+                                         : ((size_t) x*ny + y)*nz + z;
+                    sum += mVelocity[idx];
+                }
+            }
+        }
+        return static_cast<T>(8.0/sum);
+    }
+    void clear() noexcept
+    {
+        mGeometry.clear();
+        mVelocity.clear();
+        mInitialized = mHave = false;
+    }
+    // Used by Solver3D.
+    [[nodiscard]] const std::vector<double> &velocities() const noexcept
+    { return mVelocity; }
+    [[nodiscard]] bool nodal() const noexcept { return mNodal; }
+    [[nodiscard]] Ordering3D ordering() const noexcept { return mOrdering; }
+private:
+    template <typename U>
+    void set(int n, int nExpected, const U *v, Ordering3D ordering,
+             bool nodal, const char *what)
+    {
+        if (n != nExpected)
+        {
+            throw std::invalid_argument(std::string(what) + " = " +
+                                        std::to_string(n) + " must equal " +
+                                        std::to_string(nExpected));
+        }
+        if (v == nullptr) { throw std::invalid_argument("velocities is NULL"); }
+        std::vector<double> copy(v, v + n);
+        for (double x : copy)
+        {
+            if (!(x > 0))
+            {
+                throw std::invalid_argument("Non-positive velocities "
+                                            "detected");
+            }
+        }
+        mVelocity = std::move(copy);
+        mOrdering = ordering;
+        mNodal = nodal;
+        mHave = true;
+    }
+    Geometry3D mGeometry;
+    std::vector<double> mVelocity;
+    Ordering3D mOrdering = Ordering3D::Natural;
+    bool mNodal = true, mInitialized = false, mHave = false;
+};
+
+template <class T>
+class Model2D
+{
+public:
+    void initialize(const Geometry2D &geometry)
+    {
+        (void) geometry.c();
+        mGeometry = geometry;
+        mInitialized = true;
+        mVelocity.clear();
+        mHave = false;
+    }
+    [[nodiscard]] bool isInitialized() const noexcept { return mInitialized; }
+    [[nodiscard]] Geometry2D getGeometry() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry;
+    }
+    [[nodiscard]] int getNumberOfGridPoints() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry.getNumberOfGridPoints();
+    }
+    [[nodiscard]] int getNumberOfCells() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry.getNumberOfCells();
+    }
+    void setNodalVelocities(int nGrid, const T velocity[],
+                            Ordering2D ordering)
+    {
+        if (nGrid != getNumberOfGridPoints())
+        {
+            throw std::invalid_argument("Number of grid points = " +
+                                        std::to_string(nGrid) +
+                                        " must equal " +
+                                        std::to_string(
+                                            getNumberOfGridPoints()));
+        }
+        if (velocity == nullptr)
+        {
+            throw std::invalid_argument("velocities is NULL");
+        }
+        std::vector<double> copy(velocity, velocity + nGrid);
+        for (double x : copy)
+        {
+            if (!(x > 0))
+            {
+                throw std::invalid_argument("Non-positive velocities "
+                                            "detected");
+            }
+        }
+        mVelocity = std::move(copy);
+        mOrdering = ordering;
+        mHave = true;
+    }
+    [[nodiscard]] bool haveVelocities() const noexcept { return mHave; }
+    void clear() noexcept
+    {
+        mGeometry.clear();
+        mVelocity.clear();
+        mInitialized = mHave = false;
+    }
+    [[nodiscard]] const std::vector<double> &velocities() const noexcept
+    { return mVelocity; }
+    [[nodiscard]] Ordering2D ordering() const noexcept { return mOrdering; }
+private:
+    Geometry2D mGeometry;
+    std::vector<double> mVelocity;
+    Ordering2D mOrdering = Ordering2D::Natural;
+    bool mInitialized = false, mHave = false;
+};
+
+// ---------------------------------------------------------------------------
+// Solver3D<T> (include/eikonalxx/solver3d.hpp:19-177, src/solver3d.cpp)
+// ---------------------------------------------------------------------------
+template <class T>
+class Solver3D
+{
+public:
+    Solver3D() { detail::check(ekx_solver3d_create(&mHandle)); }
+    ~Solver3D() { ekx_solver3d_destroy(mHandle); }
+    Solver3D(const Solver3D &) = delete;
+    Solver3D &operator=(const Solver3D &) = delete;
+
+    void initialize(const SolverOptions &options, const Geometry3D &geometry)
+    {
+        ekx_geometry3d g = geometry.c();
+        detail::check(ekx_solver3d_initialize(mHandle, &options.c(), &g));
+        mOptions = options;
+        mGeometry = geometry;
+        mStations.clear();
+        mSource = Source3D{};
+        mField.clear();
+    }
+    [[nodiscard]] SolverOptions getOptions() const
+    {
+        detail::require(isInitialized(), "Class not initialized");
+        return mOptions;
+    }
+    [[nodiscard]] Geometry3D getGeometry() const
+    {
+        detail::require(isInitialized(), "Class not initialized");
+        return mGeometry;
+    }
+    [[nodiscard]] bool isInitialized() const noexcept
+    {
+        return ekx_solver3d_is_initialized(mHandle) != 0;
+    }
+    // src/solver3d.cpp:212-249
+    void setVelocityModel(const Model3D<T> &model)
+    {
+        detail::require(isInitialized(), "Class not initialized");
+        if (!model.haveVelocities())
+        {
+            throw std::invalid_argument("Velocity model not initialized");
+        }
+        const std::vector<double> &v = model.velocities();
+        const int ordering = static_cast<int>(model.ordering());
+        if (model.nodal())
+        {
+            detail::check(ekx_solver3d_set_nodal_velocities(
+                mHandle, (int64_t) v.size(), v.data(), ordering));
+        }
+        else
+        {
+            detail::check(ekx_solver3d_set_cell_velocities(
+                mHandle, (int64_t) v.size(), v.data(), ordering));
+        }
+        mModel = model;
+        mField.clear();
+    }
+    [[nodiscard]] bool haveVelocityModel() const noexcept
+    {
+        return ekx_solver3d_have_velocity_model(mHandle) != 0;
+    }
+    [[nodiscard]] T getSlowness(int iCellX, int iCellY, int iCellZ) const
+    {
+        detail::require(haveVelocityModel(), "Velocity model not set");
+        return mModel.getSlowness(iCellX, iCellY, iCellZ);
+    }
+    void setSource(const Source3D &source)
+    {
+        setSource(std::make_tuple(source.getLocationInX(),
+                                  source.getLocationInY(),
+                                  source.getLocationInZ()));
+    }
+    void setSource(const std::tuple<double, double, double> &location)
+    {
+        detail::require(isInitialized(), "Class not initialized");
+        detail::check(ekx_solver3d_set_source(mHandle,
+                                              std::get<0>(location),
+                                              std::get<1>(location),
+                                              std::get<2>(location)));
+        Source3D s;
+        s.setGeometry(mGeometry);
+        s.setLocationInX(std::get<0>(location));
+        s.setLocationInY(std::get<1>(location));
+        s.setLocationInZ(std::get<2>(location));
+        mSource = s;
+        mField.clear();
+    }
+    [[nodiscard]] Source3D getSource() const
+    {
+        detail::require(haveSource(), "Source location not set");
+        return mSource;
+    }
+    [[nodiscard]] bool haveSource() const noexcept
+    {
+        return ekx_solver3d_have_source(mHandle) != 0;
+    }
+    // Stored only, as src/solver3d.cpp:555-609 does.
+    void setStations(const std::vector<Station3D> &stations)
+    {
+        detail::require(isInitialized(), "Class not initialized");
+        mStations = stations;
+    }
+    [[nodiscard]] std::vector<Station3D> getStations() const
+    {
+        return mStations;
+    }
+    void solve()
+    {
+        detail::check(ekx_solver3d_solve(mHandle));
+        mField.clear();
+    }
+    [[nodiscard]] bool haveTravelTimeField() const noexcept
+    {
+        return ekx_solver3d_have_travel_time_field(mHandle) != 0;
+    }
+    [[nodiscard]] std::vector<T> getTravelTimeField() const
+    {
+        detail::require(haveTravelTimeField(),
+                        "Travel time field not yet computed");
+        const int64_t n = mGeometry.getNumberOfGridPoints();
+        std::vector<T> out((size_t) n);
+        if constexpr (std::is_same_v<T, float>)
+        {
+            detail::check(ekx_solver3d_get_travel_times_f32(mHandle, n,
+                                                            out.data()));
+        }
+        else
+        {
+            detail::check(ekx_solver3d_get_travel_times(mHandle, n,
+                                                        out.data()));
+        }
+        return out;
+    }
+    // Host copy owned by the solver, valid until the next solve
+    // (src/solver3d.cpp:516-525).
+    [[nodiscard]] const T *getTravelTimeFieldPointer() const
+    {
+        if (mField.empty()) { mField = getTravelTimeField(); }
+        return mField.data();
+    }
+    [[nodiscard]] ekx_solve_stats getStatistics() const
+    {
+        ekx_solve_stats st{};
+        detail::check(ekx_solver3d_get_stats(mHandle, &st));
+        return st;
+    }
+    void clear() noexcept
+    {
+        ekx_solver3d_destroy(mHandle);
+        mHandle = nullptr;
+        (void) ekx_solver3d_create(&mHandle);
+        mField.clear();
+        mStations.clear();
+        mModel.clear();
+    }
+private:
+    ekx_solver3d *mHandle = nullptr;
+    SolverOptions mOptions;
+    Geometry3D mGeometry;
+    Model3D<T> mModel;
+    Source3D mSource;
+    std::vector<Station3D> mStations;
+    mutable std::vector<T> mField;
+};
+
+// ---------------------------------------------------------------------------
+// Solver2D<T> (include/eikonalxx/solver2d.hpp:19-196, src/solver2d.cpp)
+// ---------------------------------------------------------------------------
+template <class T>
+class Solver2D
+{
+public:
+    Solver2D() { detail::check(ekx_solver2d_create(&mHandle)); }
+    ~Solver2D() { ekx_solver2d_destroy(mHandle); }
+    Solver2D(const Solver2D &) = delete;
+    Solver2D(Solver2D &&) = delete;
+    Solver2D &operator=(const Solver2D &) = delete;
+
+    void initialize(const SolverOptions &options, const Geometry2D &geometry)
+    {
+        ekx_geometry2d g = geometry.c();
+        detail::check(ekx_solver2d_initialize(mHandle, &options.c(), &g));
+        mOptions = options;
+        mGeometry = geometry;
+        mInitialized = true;
+        mHaveModel = mHaveSource = mHaveField = false;
+        mStations.clear();
+        mField.clear();
+    }
+    [[nodiscard]] SolverOptions getOptions() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mOptions;
+    }
+    [[nodiscard]] Geometry2D getGeometry() const
+    {
+        detail::require(mInitialized, "Class not initialized");
+        return mGeometry;
+    }
+    [[nodiscard]] bool isInitialized() const noexcept { return mInitialized; }
+    void setVelocityModel(const Model2D<T> &model)
+    {
+        detail::require(mInitialized, "Class not initialized");
+        if (!model.haveVelocities())
+        {
+            throw std::invalid_argument("Velocity model not initialized");
+        }
+        const std::vector<double> &v = model.velocities();
+        detail::check(ekx_solver2d_set_nodal_velocities(
+            mHandle, (int64_t) v.size(), v.data(),
+            static_cast<int>(model.ordering())));
+        mHaveModel = true;
+        mHaveField = false;
+    }
+    [[nodiscard]] bool haveVelocityModel() const noexcept
+    {
+        return mHaveModel;
+    }
+    void setSource(const Source2D &source)
+    {
+        setSource(std::make_pair(source.getLocationInX(),
+                                 source.getLocationInZ()));
+    }
+    void setSource(const std::pair<double, double> &location)
+    {
+        detail::require(mInitialized, "Class not initialized");
+        detail::check(ekx_solver2d_set_source(mHandle, location.first,
+                                              location.second));
+        Source2D s;
+        s.setGeometry(mGeometry);
+        s.setLocationInX(location.first);
+        s.setLocationInZ(location.second);
+        mSource = s;
+        mHaveSource = true;
+        mHaveField = false;
+    }
+    [[nodiscard]] Source2D getSource() const
+    {
+        detail::require(mHaveSource, "Source location not set");
+        return mSource;
+    }
+    [[nodiscard]] bool haveSource() const noexcept { return mHaveSource; }
+    void setStations(const std::vector<Station2D> &stations)
+    {
+        detail::require(mInitialized, "Class not initialized");
+        std::vector<double> x, z;
+        for (const auto &s : stations)
+        {
+            x.push_back(s.getLocationInX());
+            z.push_back(s.getLocationInZ());
+        }
+        detail::check(ekx_solver2d_set_stations(mHandle, (int) x.size(),
+                                                x.data(), z.data()));
+        mStations = stations;
+        mHaveField = false;
+    }
+    [[nodiscard]] std::vector<Station2D> getStations() const
+    {
+        return mStations;
+    }
+    void solve()
+    {
+        detail::check(ekx_solver2d_solve(mHandle));
+        mHaveField = true;
+        mField.clear();
+    }
+    [[nodiscard]] bool haveTravelTimeField() const noexcept
+    {
+        return mHaveField;
+    }
+    [[nodiscard]] std::vector<T> getTravelTimeField() const
+    {
+        detail::require(mHaveField, "Travel time field not yet computed");
+        const int64_t n = mGeometry.getNumberOfGridPoints();
+        std::vector<double> out((size_t) n);
+        detail::check(ekx_solver2d_get_travel_times(mHandle, n, out.data()));
+        return std::vector<T>(out.begin(), out.end());
+    }
+    [[nodiscard]] const T *getTravelTimeFieldPointer() const
+    {
+        if (mField.empty()) { mField = getTravelTimeField(); }
+        return mField.data();
+    }
+    // src/solver2d.cpp:106-167 (bilinear interpolation at the stations)
+    [[nodiscard]] std::vector<T> getTravelTimesAtStations() const
+    {
+        detail::require(mHaveField, "Travel time field not yet computed");
+        std::vector<double> out(mStations.size());
+        if (!out.empty())
+        {
+            detail::check(ekx_solver2d_get_travel_times_at_stations(
+                mHandle, (int) out.size(), out.data()));
+        }
+        return std::vector<T>(out.begin(), out.end());
+    }
+    [[nodiscard]] ekx_solve_stats getStatistics() const
+    {
+        ekx_solve_stats st{};
+        detail::check(ekx_solver2d_get_stats(mHandle, &st));
+        return st;
+    }
+private:
+    ekx_solver2d *mHandle = nullptr;
+    SolverOptions mOptions;
+    Geometry2D mGeometry;
+    Source2D mSource;
+    std::vector<Station2D> mStations;
+    bool mInitialized = false, mHaveModel = false, mHaveSource = false;
+    bool mHaveField = false;
+    mutable std::vector<T> mField;
+};
+
+} // namespace eikonalxx_amd
+
+#ifndef EKX_NO_EIKONALXX_ALIAS
+namespace EikonalXX = eikonalxx_amd;
+#endif

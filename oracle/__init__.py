@@ -396,3 +396,56 @@ def grid_to_level_index2d(sweep, ix, iz, nx, nz):
     lib().ekxo_grid_to_level_index2d(sweep, ix, iz, nx, nz,
                                      *[ctypes.byref(x) for x in v])
     return v[0].value, v[1].value
+
+
+def number_of_levels2d(nx, nz):
+    return lib().ekxo_number_of_levels2d(nx, nz)
+
+
+def graph3d_number_of_levels(nx, ny, nz):
+    return lib().ekxo_graph3d_number_of_levels(nx, ny, nz)
+
+
+def sweep_level_index_to_grid2d(sweep, level, indx, nx, nz):
+    v = _ints(2)
+    lib().ekxo_sweep_level_index_to_grid2d(sweep, level, indx, nx, nz,
+                                           *[ctypes.byref(x) for x in v])
+    return v[0].value, v[1].value
+
+
+def travel_time_indices2d(sweep, ix, iz, nx):
+    out = np.zeros(4, dtype=np.int32)
+    lib().ekxo_travel_time_indices2d(sweep, ix, iz, nx, _ip(out))
+    return out
+
+
+def analytic_time3d(ijk, dxyz, src, slow):
+    return lib().ekxo_analytic_time3d(ijk[0], ijk[1], ijk[2], dxyz[0],
+                                      dxyz[1], dxyz[2], src[0], src[1],
+                                      src[2], slow)
+
+
+def analytic_time_deriv3d(ijk, dxyz, src, slow):
+    v = [ctypes.c_double(0.0) for _ in range(4)]
+    lib().ekxo_analytic_time_deriv3d(ijk[0], ijk[1], ijk[2], dxyz[0],
+                                     dxyz[1], dxyz[2], src[0], src[1],
+                                     src[2], slow,
+                                     *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)   # t, dt/dx, dt/dy, dt/dz
+
+
+def analytic_time2d(ik, dxz, src, slow):
+    return lib().ekxo_analytic_time2d(ik[0], ik[1], dxz[0], dxz[1], src[0],
+                                      src[1], slow)
+
+
+def fd2d_nonuniform(radius, dxz, s_src, ik, signs, shifts, isrc, src_off,
+                    s3, t3, huge=1.0e10):
+    sg = (ctypes.c_int * 2)(*signs)
+    sh = (ctypes.c_int * 2)(*shifts)
+    isr = (ctypes.c_int * 2)(*isrc)
+    off = (ctypes.c_double * 2)(*src_off)
+    s = (ctypes.c_double * 3)(*s3)
+    t = (ctypes.c_double * 3)(*t3)
+    return lib().ekxo_fd2d_nonuniform(radius, huge, dxz[0], dxz[1], s_src,
+                                      ik[0], ik[1], sg, sh, isr, off, s, t)

@@ -93,3 +93,60 @@ def test_batch_polish_matches_single(ekx, mixed_case):
         s.solve()
         assert np.array_equal(b.get_travel_time_field(i),
                               s.get_travel_time_field())
+
+
+@pytest.mark.parametrize("prec", ["FIXED32", "FIXED32_POLISH"])
+def test_fixed32_within_1e5(ekx, mixed_case, prec):
+    """Scaled-integer storage with FP64 arithmetic: the stored times carry
+    at most half a code step (about 1e-9 of the travel-time bound), so the
+    field stays within the 1e-5 contract without relying on the polish."""
+    w, slow = mixed_case
+    t64 = _solve(ekx, w, slow, ekx.Precision.FP64).get_travel_time_field()
+    s = _solve(ekx, w, slow, getattr(ekx.Precision, prec))
+    t = s.get_travel_time_field()
+    _, _, idx = s.get_source_cell()
+    mx, rms = _rel(t, t64, slow[idx], w["h"])
+    st = s.get_stats()
+    print("%s: max %.3e rms %.3e" % (prec, mx, rms), st.as_dict())
+    assert mx <= 1e-5
+    assert np.all(t < 1e9)                 # every node reached
+    if prec == "FIXED32":
+        assert st.updates_fp64 == 0
+    else:
+        assert 0 < st.updates_fp64 < st.updates
+
+
+def test_fixed32_kernels_agree(ekx, mixed_case):
+    """Column and hyperplane kernels run the same FP64 arithmetic on the
+    same codes in a topological order: identical fields."""
+    w, slow = mixed_case
+    a = _solve(ekx, w, slow, ekx.Precision.FIXED32, ekx.Kernel.COLUMN, 1)
+    b = _solve(ekx, w, slow, ekx.Precision.FIXED32, ekx.Kernel.HYPERPLANE, 1)
+    assert np.array_equal(a.get_travel_time_field(),
+                          b.get_travel_time_field())
+
+
+def test_fixed32_device_pointer_refused(ekx, mixed_case):
+    w, slow = mixed_case
+    s = _solve(ekx, w, slow, ekx.Precision.FIXED32, n_sweeps=1)
+    with pytest.raises(RuntimeError):
+        s.get_travel_time_field_device()
+
+
+def test_fixed32_batch_matches_single(ekx, mixed_case):
+    w, slow = mixed_case
+    g = ekx.Geometry3D(w["nx"], w["ny"], w["nz"], w["h"])
+    srcs = [w["source"], (100.0, 1200.0, 0.0), (2000.0, 300.0, 1500.0)]
+    opts = ekx.SolverOptions(number_of_sweeps=2, factored_radius=w["radius"],
+                             precision=ekx.Precision.FIXED32)
+    b = ekx.Batch3D()
+    b.initialize(opts, [(g, slow, s) for s in srcs])
+    b.solve()
+    for i, src in enumerate(srcs):
+        s = ekx.Solver3D()
+        s.initialize(opts, g)
+        s.set_slowness(slow)
+        s.set_source(src)
+        s.solve()
+        assert np.array_equal(b.get_travel_time_field(i),
+                              s.get_travel_time_field())

@@ -1,0 +1,326 @@
+"""The oracle's local solvers and sweep schedules.
+
+What pins them besides their line-by-line restatement:
+  * exactness of the Cartesian operators on planar fronts (the 1D head-wave,
+    2D and 8-point operators reproduce T = s n.x to rounding);
+  * the non-uniform restatement (solverUtilities3d.hpp:562-824) against the
+    uniform one (:258-558) on equal spacings -- two independently written
+    code paths of the reference;
+  * FSM invariants: a sweep never raises a travel time, and any topological
+    order of the nodes (serial loops vs hyperplanes) gives the same field;
+  * convergence of whole solves to the analytic fields, at the first order
+    of the scheme, and the reference's own end-to-end 2D case
+    (testing/solver2d.cpp:1195-1302);
+  * the travel-time bound the scaled-integer storage relies on.
+The factored operators have no closed-form special case (the reference's
+quadratics do not return T0 on exact T0 data); they are pinned only by the
+convergence tests and stay "partially pinned".
+"""
+import math
+
+import numpy as np
+import pytest
+
+HUGE = 1.0e10
+
+
+def _signs(oracle, sweep):
+    ixs, iys, izs, sx, sy, sz = oracle.sweep_signs3d(sweep)
+    return (sx, sy, sz), (ixs, iys, izs)
+
+
+def _neighbours(T, ix, iy, iz, shifts):
+    ixs, iys, izs = shifts
+    return (T(ix + ixs, iy, iz), T(ix + ixs, iy + iys, iz),
+            T(ix, iy + iys, iz), T(ix, iy, iz + izs),
+            T(ix + ixs, iy, iz + izs), T(ix + ixs, iy + iys, iz + izs),
+            T(ix, iy + iys, iz + izs))
+
+
+FAR = (1000, 1000, 1000)     # source cell far away: Cartesian branch
+
+
+@pytest.mark.parametrize("sweep", range(1, 9))
+def test_axis_plane_wave_exact(oracle, sweep):
+    """A front along an axis, travelling with the sweep: the 1D and 2D
+    operators give T exactly and the 8-point operator is gated off."""
+    h, s = 10.0, 1.0 / 3200.0
+    signs, shifts = _signs(oracle, sweep)
+    for axis in range(3):
+        def T(ix, iy, iz):
+            return s * h * signs[axis] * (ix, iy, iz)[axis]
+        for node in ((7, 9, 11), (20, 3, 5)):
+            t7 = _neighbours(T, *node, shifts)
+            got = oracle.fd3d_uniform(2, h, s, node, signs, shifts, FAR,
+                                      (0, 0, 0), (s,) * 7, t7)
+            assert got == pytest.approx(T(*node), rel=4e-16, abs=4e-16)
+            got = oracle.fd3d_nonuniform(2, (h, h, h), s, node, signs,
+                                         shifts, FAR, (0, 0, 0), (s,) * 7,
+                                         t7)
+            assert got == pytest.approx(T(*node), rel=4e-16, abs=4e-16)
+
+
+@pytest.mark.parametrize("sweep", range(1, 9))
+def test_diagonal_plane_wave_exact(oracle, sweep):
+    """A front along the cube diagonal: the 8-point (FTeik) operator gives
+    T exactly (Tx = Ty = Tz = T - 3a, det = 27 (hs)^2)."""
+    h, s = 25.0, 1.0 / 4100.0
+    signs, shifts = _signs(oracle, sweep)
+
+    def T(ix, iy, iz):
+        return s * h * (signs[0] * ix + signs[1] * iy
+                        + signs[2] * iz) / math.sqrt(3.0) + 2.0
+    node = (12, 13, 14)
+    t7 = _neighbours(T, *node, shifts)
+    got = oracle.fd3d_uniform(-1, h, s, node, signs, shifts, FAR, (0, 0, 0),
+                              (s,) * 7, t7)
+    assert got == pytest.approx(T(*node), rel=1e-15)
+    got = oracle.fd3d_nonuniform(-1, (h, h, h), s, node, signs, shifts, FAR,
+                                 (0, 0, 0), (s,) * 7, t7)
+    assert got == pytest.approx(T(*node), rel=1e-15)
+
+
+def test_nonuniform_matches_uniform_on_equal_spacing(oracle):
+    """Random causal neighbourhoods, both branches (Cartesian and factored):
+    the non-uniform operators with dx = dy = dz agree with the uniform
+    ones to rounding (1e-10 relative: the factored quadratics cancel;
+    a dropped or mis-signed term shows at 1e-4 or more)."""
+    rng = np.random.default_rng(11)
+    h = 15.0
+    checked = 0
+    for trial in range(3000):
+        sweep = int(rng.integers(1, 9))
+        signs, shifts = _signs(oracle, sweep)
+        s7 = tuple(1.0 / rng.uniform(1500, 6000, 7))
+        src_cell = (10, 10, 10)
+        off = tuple(rng.uniform(150.0, 165.0, 3))
+        node = tuple(int(v) for v in rng.integers(7, 14, 3))
+        radius = int(rng.choice([-1, 3]))
+        # neighbours from a smooth field plus noise, most of them causal
+        base = rng.uniform(1.0, 2.0)
+        t7 = tuple(base + rng.uniform(-0.004, 0.0, 7))
+        a = oracle.fd3d_uniform(radius, h, s7[0], node, signs, shifts,
+                                src_cell, off, s7, t7)
+        b = oracle.fd3d_nonuniform(radius, (h, h, h), s7[0], node, signs,
+                                   shifts, src_cell, off, s7, t7)
+        assert b == pytest.approx(a, rel=1e-10), (trial, a, b)
+        checked += a < HUGE
+    assert checked > 2000
+
+
+def test_2d_plane_waves_exact(oracle):
+    """2D: a front along x (1D and Vidale operators) and along the diagonal
+    (Vidale) give T exactly; with t2 = t3 = HUGE the update is the 1D head
+    wave t1 + dx min(s0, s3) (the case of the commented-out reference test
+    testing/solver2d.cpp:138-243)."""
+    h, s = 10.0, 1.0 / 3000.0
+    for sweep in range(1, 5):
+        ixs, izs, sx, sz = oracle.sweep_signs2d(sweep)
+        for kind in ("x", "diag"):
+            def T(ix, iz):
+                if kind == "x":
+                    return s * h * sx * ix + 1.0
+                return s * h * (sx * ix + sz * iz) / math.sqrt(2.0) + 1.0
+            ik = (9, 7)
+            t3 = (T(ik[0] + ixs, ik[1]), T(ik[0] + ixs, ik[1] + izs),
+                  T(ik[0], ik[1] + izs))
+            got = oracle.fd2d_uniform(-1, h, s, ik, (sx, sz), (ixs, izs),
+                                      (100, 100), (0, 0), (s, s, s), t3)
+            assert got == pytest.approx(T(*ik), rel=2e-16)
+    dx, dz = 100.0, 200.0
+    s0, s1, s3 = 1 / 4000.0, 1 / 4000.0, 1 / 5000.0
+    got = oracle.fd2d_nonuniform(-1, (dx, dz), s0, (5, 5), (1, 1), (-1, -1),
+                                 (100, 100), (0, 0), (s0, s1, s3),
+                                 (1.0, HUGE, HUGE))
+    assert got == pytest.approx(1.0 + dx * s3, rel=1e-15)
+    got = oracle.fd2d_nonuniform(-1, (dx, dz), s0, (5, 5), (1, 1), (-1, -1),
+                                 (100, 100), (0, 0), (s0, 1 / 4500.0, s3),
+                                 (HUGE, HUGE, 2.0))
+    assert got == pytest.approx(2.0 + dz / 4500.0, rel=1e-15)
+
+
+def _random_problem(rng, n=(17, 14, 12), h=20.0):
+    g_slow = 1.0 / rng.uniform(1500.0, 6000.0,
+                               (n[0] - 1) * (n[1] - 1) * (n[2] - 1))
+    src = tuple(rng.uniform(0.0, (m - 1) * h) for m in n)
+    return n, h, g_slow, src
+
+
+def test_sweeps_never_raise_travel_times(oracle):
+    """private/solver3d.hpp:236-240 stores a candidate only if smaller."""
+    rng = np.random.default_rng(5)
+    n, h, slow, src = _random_problem(rng)
+    g = oracle.geometry3d(*n, h)
+    T = oracle.initialize_travel_times3d(g, slow, src)
+    prev = T.copy()
+    for it in range(3):
+        for sweep in range(1, 9):
+            oracle.sweep3d(g, slow, src, 3, sweep, it == 0, T)
+            assert np.all(T <= prev)
+            prev = T.copy()
+    assert np.all(T < HUGE)
+
+
+@pytest.mark.parametrize("radius", [-1, 2, 4])
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_serial_and_level_set_orders_agree(oracle, radius, schedule):
+    """A node reads only upwind neighbours, which lie on earlier
+    hyperplanes, so the level-set order reproduces the serial loops bit for
+    bit (the fact the GPU kernels are built on)."""
+    rng = np.random.default_rng(100 + radius)
+    n, h, slow, src = _random_problem(rng, n=(19, 13, 16), h=12.5)
+    g = oracle.geometry3d(*n, h)
+    a = oracle.solve3d(g, slow, src, 2, radius, schedule=schedule)
+    b = oracle.solve3d_level_set(g, slow, src, 2, radius, schedule=schedule)
+    assert np.array_equal(a, b)
+
+
+def test_shipped_schedules(oracle):
+    """src/solver3d.cpp:451-499: the FSM runs initialisation sweeps 1 and 2
+    and returns; the level-set path returns after the source
+    initialisation (only the 8 source-cell corners are set)."""
+    rng = np.random.default_rng(8)
+    n, h, slow, src = _random_problem(rng)
+    g = oracle.geometry3d(*n, h)
+    init = oracle.initialize_travel_times3d(g, slow, src)
+    lsm = oracle.solve3d(g, slow, src, 3, 3, algorithm=0, schedule=1)
+    assert np.array_equal(lsm, init)
+    assert np.count_nonzero(lsm < HUGE) == 8
+    T = init.copy()
+    oracle.sweep3d(g, slow, src, 3, 1, True, T)
+    oracle.sweep3d(g, slow, src, 3, 2, True, T)
+    fsm = oracle.solve3d(g, slow, src, 3, 3, algorithm=1, schedule=1)
+    assert np.array_equal(fsm, T)
+
+
+def test_initialisation(oracle):
+    """src/solver3d.cpp:21-84: HUGE everywhere except the 8 corners of the
+    source cell, which hold the source-cell slowness times the distance."""
+    rng = np.random.default_rng(2)
+    n, h, slow, src = _random_problem(rng)
+    g = oracle.geometry3d(*n, h)
+    T = oracle.initialize_travel_times3d(g, slow, src).reshape(n[::-1])
+    off, cell, idx = oracle.source3d(g, *src)
+    for iz in range(n[2]):
+        for iy in range(n[1]):
+            for ix in range(n[0]):
+                corner = all(0 <= v - c <= 1
+                             for v, c in zip((ix, iy, iz), cell))
+                if corner:
+                    d = math.sqrt((ix * h - off[0]) ** 2
+                                  + (iy * h - off[1]) ** 2
+                                  + (iz * h - off[2]) ** 2)
+                    assert T[iz, iy, ix] == pytest.approx(slow[idx] * d,
+                                                          rel=1e-15)
+                else:
+                    assert T[iz, iy, ix] == HUGE
+
+
+@pytest.mark.parametrize("radius", [-1, 3])
+def test_homogeneous3d_converges(oracle, radius):
+    """Whole 3D solves against |x - xs| / v: first-order convergence
+    (halving h halves the error) for both the Cartesian and the factored
+    operators."""
+    v = 3000.0
+    errs = []
+    for n, h in ((21, 40.0), (41, 20.0), (81, 10.0)):
+        g = oracle.geometry3d(n, n, n, h)
+        L = (n - 1) * h
+        src = (0.37 * L + 3.1, 0.52 * L - 2.3, 0.44 * L + 1.7)
+        T = oracle.solve3d_level_set(g, np.full((n - 1) ** 3, 1 / v), src, 2,
+                                     radius)
+        errs.append(np.abs(T - oracle.homogeneous3d(g, src, v)).max())
+    assert errs[0] < 2e-2
+    for coarse, fine in zip(errs, errs[1:]):
+        assert 1.4 < coarse / fine < 2.8
+    assert errs[0] / errs[2] > 3.0
+
+
+def test_linear_gradient3d_converges(oracle):
+    """The 3D gradient workload shape (v = 3000 + 0.5 z, source on a node)
+    at two resolutions against the arccosh field."""
+    errs = []
+    for n, h in ((33, 40.0), (65, 20.0)):
+        g = oracle.geometry3d(n, n, n, h)
+        L = (n - 1) * h
+        v_top, v_bot = 3000.0, 3000.0 + 0.5 * L
+        src = (0.5 * L, 0.5 * L, 0.15625 * L)
+        z = (np.arange(n - 1) + 0.5) * h
+        vz = np.empty((n - 1, n - 1, n - 1))
+        # nodal velocities averaged over the 8 corners of each cell
+        vn = v_top + 0.5 * np.arange(n) * h
+        vz[:] = (0.5 * (vn[:-1] + vn[1:]))[:, None, None]
+        slow = (1.0 / vz).ravel()
+        T = oracle.solve3d_level_set(g, slow, src, 3, 3)
+        ref = oracle.linear_gradient3d(g, src, v_top, v_bot)
+        errs.append(np.abs(T - ref).max())
+        assert np.all(z > 0)
+    assert errs[1] < errs[0]
+    assert errs[1] < 6e-3
+
+
+def test_2d_config_homogeneous(oracle):
+    """201 x 201, h = 10 m, v = 5000 m/s, source (1005, 995), radius 3,
+    2 iterations, against |x - xs| / v."""
+    n, h, v = 201, 10.0, 5000.0
+    g = oracle.geometry2d(n, n, h)
+    slow = oracle.nodal_to_cell2d(n, n, np.full(n * n, v))
+    T = oracle.solve2d(g, slow, (1005.0, 995.0), 2, 3)
+    err = np.abs(T - oracle.homogeneous2d(g, (1005.0, 995.0), v))
+    assert err.max() < 1e-3
+    assert err.mean() < 3e-4
+
+
+def test_2d_converges(oracle):
+    """testing/converge2d.cpp: the 2D error falls with the grid spacing."""
+    v, width = 1000.0, 2000.0
+    errs = []
+    for dx in (40.0, 20.0, 10.0):
+        n = int(round(width / dx)) + 1
+        g = oracle.geometry2d(n, n, dx)
+        src = (dx / 2, 0.0)
+        T = oracle.solve2d(g, np.full((n - 1) ** 2, 1 / v), src, 2, 3)
+        errs.append(np.mean(np.abs(T - oracle.homogeneous2d(g, src, v))))
+    assert errs[0] > errs[1] > errs[2]
+
+
+def test_2d_reference_stations(oracle):
+    """testing/solver2d.cpp:1195-1302: a station at the source reads ~0 and
+    another station equals the hand-computed bilinear interpolation."""
+    nx, nz, dx, dz, vel = 21, 20, 100.0, 100.05, 5000.0
+    xs, zs = dx * (nx // 2) + dx / 4, dz * (nz // 2)
+    g = oracle.geometry2d(nx, nz, dx, dz)
+    slow = oracle.nodal_to_cell2d(nx, nz, np.full(nx * nz, vel))
+    T = oracle.solve2d(g, slow, (xs, zs), 0, 3)
+    assert abs(oracle.station_time2d(g, slow, T, (xs, zs), (xs, zs))) < 1e-7
+    lx, lz = xs + 2 * dx, zs + 3.1 * dz
+    cx, cz = int(lx / dx), int(lz / dz)
+    i00 = cz * nx + cx
+    t00, t10, t01, t11 = T[i00], T[i00 + 1], T[i00 + nx], T[i00 + nx + 1]
+    f0 = ((cx + 1) * dx - lx) / dx * t00 + (lx - cx * dx) / dx * t10
+    f1 = ((cx + 1) * dx - lx) / dx * t01 + (lx - cx * dx) / dx * t11
+    fi = ((cz + 1) * dz - lz) / dz * f0 + (lz - cz * dz) / dz * f1
+    assert abs(fi - oracle.station_time2d(g, slow, T, (xs, zs),
+                                          (lx, lz))) < 1e-8
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_travel_time_bound(oracle, seed):
+    """The bound the scaled-integer storage is sized by: every finite travel
+    time is at most sMax times the largest Manhattan distance from the
+    source to a node, plus 8 cells."""
+    rng = np.random.default_rng(40 + seed)
+    n = (int(rng.integers(8, 20)), int(rng.integers(8, 20)),
+         int(rng.integers(8, 20)))
+    d = tuple(rng.uniform(5.0, 30.0, 3))
+    slow = 1.0 / rng.uniform(300.0, 8000.0,
+                             (n[0] - 1) * (n[1] - 1) * (n[2] - 1))
+    src = tuple(rng.uniform(0.0, (m - 1) * dd) for m, dd in zip(n, d))
+    g = oracle.geometry3d(*n, *d)
+    for n_sweeps, radius in ((0, 3), (2, -1), (3, 2)):
+        T = oracle.solve3d_level_set(g, slow, src, n_sweeps, radius)
+        ext = sum(max(sv, (m - 1) * dd - sv)
+                  for sv, m, dd in zip(src, n, d))
+        bound = slow.max() * (ext + 8 * max(d))
+        assert np.all(T < HUGE)
+        assert T.max() <= bound
