@@ -63,6 +63,8 @@ def build_tables_workload(args):
     from eikonalxx_amd import workloads
     w = workloads.config5_tables(nx=args.nx, ny=args.ny, nz=args.nz,
                                  h=args.h, n_sources=args.sources)
+    if args.iterations is not None:
+        w["n_sweeps"] = args.iterations
     slow = workloads.cell_slowness(w["velocity"], w["nx"], w["ny"], w["nz"])
     return w, slow
 
@@ -88,6 +90,25 @@ def cpu_baseline(w, slow, source, threads, iterations):
         upd += (nx - j0[0]) * (ny - j0[1]) * (nz - j0[2])
     upd += iterations * 8 * (nx - 1) * (ny - 1) * (nz - 1)
     return upd / dt, dt, upd
+
+
+def measured_traffic(kernel, per_launch, args):
+    """HBM bytes per launch of `kernel`, from the rocprofv3 counter summary
+    under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    the same counter's reading for a copy of known size, and expressed per
+    algorithmic byte so that it applies to this run's launch sizes).  None
+    when no summary covers the kernel."""
+    path = os.path.join(ROOT, "profiles", args.traffic_file)
+    if not os.path.exists(path):
+        return None
+    ratio = None
+    for line in open(path):
+        # "kernel <name> hbm_per_algorithmic_byte <r>"
+        parts = line.split()
+        if (len(parts) == 4 and parts[0] == "kernel" and parts[1] == kernel
+                and parts[2] == "hbm_per_algorithmic_byte"):
+            ratio = float(parts[3])
+    return None if ratio is None else round(ratio * per_launch)
 
 
 def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
@@ -116,28 +137,20 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
         else:
             bulk_ms.append(st.sweep_ms_fp32)
             pol_ms.append(st.sweep_ms_fp64)
+        log("%s step %d: %.1f ms" % (mode, i, st.last_solve_ms))
     ekx.synchronize()
     wall = time.perf_counter() - wall0
     st = batch.get_stats()
     ms_per_step = 1000.0 * wall / args.steps
     value = st.updates / (wall / args.steps)
     if nbytes == 8:
-        bulk_updates, bulk_launches = st.updates, st.kernel_launches
+        bulk_updates = st.updates
     else:
         bulk_updates = st.updates - st.updates_fp64
-        bulk_launches = st.kernel_launches - st.kernel_launches_fp64
     bulk = float(np.mean(bulk_ms))
-    achieved = bulk_updates * BYTES_PER_UPDATE[nbytes] / (bulk / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": kernel,
-                "bytes_per_update": BYTES_PER_UPDATE[nbytes],
-                "updates_per_step": bulk_updates,
-                "launches_per_step": bulk_launches,
-                "avg_launch_us": round(1000.0 * bulk / bulk_launches, 2)}
+    phase_gbs = bulk_updates * BYTES_PER_UPDATE[nbytes] / (bulk / 1e3) / 1e9
     nx, ny, nz, h = w["nx"], w["ny"], w["nz"], w["h"]
-    max_err, rms_acc = 0.0, 0.0
+    max_err, rms_acc = (0.0, 0.0) if picks else (None, None)
     for k, i in enumerate(picks):
         t = batch.get_travel_time_field(i)
         x, y, z = w["sources"][i]
@@ -149,19 +162,79 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
         max_err = max(max_err, mx)
         rms_acc += rms * rms
     del batch
+
+    # The dominant kernel on its own: one more solve with eager launches,
+    # each bulk column launch bracketed by HIP events on its stream.
+    if not args.no_kernel_timing:
+        topts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
+                                  factored_radius=w["radius"],
+                                  precision=getattr(ekx.Precision, mode),
+                                  kernel_timing=True)
+        tb = ekx.Batch3D()
+        tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
+        tb.solve()
+        ts = tb.get_stats()
+        del tb
+        log("%s timed solve: %d bulk launches, %.1f ms of kernels"
+            % (mode, ts.timed_launches, ts.timed_kernel_ms))
+        kernel_bytes = ts.timed_updates * BYTES_PER_UPDATE[nbytes]
+        kernel_ms, launches = ts.timed_kernel_ms, ts.timed_launches
+    else:
+        kernel_bytes = bulk_updates * BYTES_PER_UPDATE[nbytes]
+        kernel_ms = bulk
+        launches = (st.kernel_launches if nbytes == 8 else
+                    st.kernel_launches - st.kernel_launches_fp64)
+    achieved = kernel_bytes / (kernel_ms / 1e3) / 1e9
+    avg_us = 1000.0 * kernel_ms / launches
+    per_launch = kernel_bytes / launches
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kernel, per_launch, args),
+                "kernel": kernel,
+                "bytes_per_update": BYTES_PER_UPDATE[nbytes],
+                "algorithmic_bytes_per_launch": round(per_launch),
+                "launches_timed": launches,
+                "avg_launch_us": round(avg_us, 2),
+                "sweep_phase_GBs": round(phase_gbs, 1)}
     out = {"updates_per_s": value, "ms_per_step": ms_per_step,
            "device_ms_per_step": float(np.mean(dev_ms)),
            "bulk_sweep_ms": bulk,
            "polish_sweep_ms": float(np.mean(pol_ms)) if pol_ms else 0.0,
            "updates_per_step": st.updates,
            "updates_fp64_per_step": st.updates_fp64,
+           "updates_factored_per_step": st.updates_factored,
            "max_rel_err_vs_fp64": max_err,
-           "rms_rel_err_vs_fp64": float(np.sqrt(rms_acc / len(picks))),
+           "rms_rel_err_vs_fp64": (float(np.sqrt(rms_acc / len(picks)))
+                                   if picks else None),
            "dtype": arith, "description": text, "roofline": roofline}
-    log("%s: %.1f ms/step, %.3e updates/s, %s %.1f GB/s, max err %.2e "
-        "rms %.2e" % (mode, ms_per_step, value, kernel, achieved, max_err,
-                      out["rms_rel_err_vs_fp64"]))
+    log("%s: %.1f ms/step, %.3e updates/s, %s %.1f GB/s (avg launch "
+        "%.1f us; whole sweep phase %.1f GB/s), max err %s rms %s"
+        % (mode, ms_per_step, value, kernel, achieved, avg_us, phase_gbs,
+           max_err, out["rms_rel_err_vs_fp64"]))
     return out
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime the
+    solver library links (its copy kernel reads and writes exactly that
+    many bytes), repeated 3 times, for scaling the HBM counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) != 0 or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) != 0:
+        print("hipMalloc failed", file=sys.stderr)
+        return 1
+    rc = hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        rc |= hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)  # D2D
+    rc |= hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("copied %d MiB device-to-device 3 times (rc %d)" % (mib, rc))
+    return 0 if rc == 0 else 1
 
 
 def main():
@@ -174,6 +247,9 @@ def main():
     ap.add_argument("--ny", type=int, default=385)
     ap.add_argument("--nz", type=int, default=193)
     ap.add_argument("--h", type=float, default=50.0)
+    ap.add_argument("--iterations", type=int, default=None,
+                    help="refinement iterations (default: the workload's "
+                         "4); fewer keeps counter-collection runs short")
     ap.add_argument("--modes", default="FP64,FIXED32,FP32_POLISH",
                     help="comma-separated precision modes to time; the "
                          "headline is the fastest within the error bar")
@@ -181,6 +257,22 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-iterations", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-file", default="traffic_sweep_columns3d.txt",
+                    help="counter summary under profiles/ giving HBM "
+                         "traffic per algorithmic byte")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    metavar="MIB",
+                    help="first copy MIB MiB device-to-device 3 times (a "
+                         "plain copy kernel of known size, the scale for "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the eager per-launch timing solve (the "
+                         "roofline then uses the whole sweep phase)")
+    ap.add_argument("--skip-checks", action="store_true",
+                    help="time the batched solves only: no accuracy check "
+                         "against FP64 and no plain single-source solve "
+                         "(for profiling runs that should hold only the "
+                         "headline workload's kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU "
@@ -198,6 +290,9 @@ def main():
     if ekx.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    if args.copy_calibration > 0 and copy_calibration(
+            args.copy_calibration) != 0:
+        return 1
 
     t0 = time.time()
     w, slow = build_tables_workload(args)
@@ -213,11 +308,16 @@ def main():
     ref_opts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
                                  factored_radius=w["radius"],
                                  precision=ekx.Precision.FP64)
-    ref = ekx.Batch3D()
-    ref.initialize(ref_opts, [(geom, slow, w["sources"][i]) for i in picks])
-    ref.solve()
-    ref_fields = [ref.get_travel_time_field(k) for k in range(len(picks))]
-    del ref
+    if args.skip_checks:
+        picks, ref_fields = [], []
+    else:
+        ref = ekx.Batch3D()
+        ref.initialize(ref_opts, [(geom, slow, w["sources"][i])
+                                  for i in picks])
+        ref.solve()
+        ref_fields = [ref.get_travel_time_field(k)
+                      for k in range(len(picks))]
+        del ref
 
     # ---- the batched solve in each precision mode ---------------------------
     results = {}
@@ -225,30 +325,33 @@ def main():
         results[m] = run_mode(ekx, m, w, slow, geom, picks, ref_fields, args)
     del ref_fields
     within = [m for m in modes
-              if results[m]["max_rel_err_vs_fp64"] <= ERROR_BAR]
+              if results[m]["max_rel_err_vs_fp64"] is not None
+              and results[m]["max_rel_err_vs_fp64"] <= ERROR_BAR]
     pool = within if within else modes
     head = max(pool, key=lambda m: results[m]["updates_per_s"])
     hr = results[head]
 
     # ---- the plain FP64 solve (one source) ---------------------------------
-    plain = ekx.Solver3D()
-    plain.initialize(ref_opts, geom)
-    plain.set_slowness(slow)
-    plain.set_source(w["sources"][picks[0]])
-    plain.solve()
-    plain_ms = []
-    for _ in range(2):
+    plain_rate = None
+    if not args.skip_checks:
+        plain = ekx.Solver3D()
+        plain.initialize(ref_opts, geom)
+        plain.set_slowness(slow)
+        plain.set_source(w["sources"][picks[0]])
         plain.solve()
-        plain_ms.append(plain.get_stats().last_solve_ms)
-    pst = plain.get_stats()
-    plain_rate = pst.updates / (min(plain_ms) / 1000.0)
-    log("plain FP64 single source: %.1f ms, %.3e updates/s"
-        % (min(plain_ms), plain_rate))
-    del plain
+        plain_ms = []
+        for _ in range(2):
+            plain.solve()
+            plain_ms.append(plain.get_stats().last_solve_ms)
+        pst = plain.get_stats()
+        plain_rate = pst.updates / (min(plain_ms) / 1000.0)
+        log("plain FP64 single source: %.1f ms, %.3e updates/s"
+            % (min(plain_ms), plain_rate))
+        del plain
 
     # ---- CPU baseline (oracle, bounded sample) ------------------------------
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.skip_checks:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         rate, secs, upd = cpu_baseline(w, slow, w["sources"][picks[0]],
                                        threads, args.cpu_iterations)
@@ -268,7 +371,8 @@ def main():
                for m, r in results.items()}
     for m, r in results.items():
         summary[m]["kernel_GBs"] = r["roofline"]["achieved"]
-        summary[m]["speedup_vs_fp64_plain"] = r["updates_per_s"] / plain_rate
+        summary[m]["speedup_vs_fp64_plain"] = (
+            r["updates_per_s"] / plain_rate if plain_rate else None)
     result = {
         "metric": "grid-pt updates/s & % HBM roofline, FP32-batched vs "
                   "FP64 plain at <=1e-5 rel err",
@@ -297,13 +401,15 @@ def main():
             "within_error_bar": head in within,
             "updates_per_step": hr["updates_per_step"],
             "updates_fp64_per_step": hr["updates_fp64_per_step"],
+            "updates_factored_per_step": hr["updates_factored_per_step"],
             "device_ms_per_step": hr["device_ms_per_step"],
             "max_rel_err_vs_fp64": hr["max_rel_err_vs_fp64"],
             "rms_rel_err_vs_fp64": hr["rms_rel_err_vs_fp64"],
             "error_sources": len(picks),
             "fp64_plain_updates_per_s": plain_rate,
             "fp64_plain_workload": "1 source, same grid and schedule, FP64",
-            "speedup_vs_fp64_plain": hr["updates_per_s"] / plain_rate,
+            "speedup_vs_fp64_plain": (hr["updates_per_s"] / plain_rate
+                                      if plain_rate else None),
             "modes": summary,
         },
         "roofline": hr["roofline"],

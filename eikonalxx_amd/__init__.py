@@ -72,7 +72,8 @@ class _Options(ctypes.Structure):
                 ("schedule", ctypes.c_int),
                 ("kernel", ctypes.c_int),
                 ("early_exit", ctypes.c_int),
-                ("use_graph", ctypes.c_int)]
+                ("use_graph", ctypes.c_int),
+                ("kernel_timing", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -92,10 +93,15 @@ class _Geometry2D(ctypes.Structure):
 class SolveStats(ctypes.Structure):
     _fields_ = [("updates", ctypes.c_int64),
                 ("updates_fp64", ctypes.c_int64),
+                ("updates_factored", ctypes.c_int64),
                 ("kernel_launches", ctypes.c_int64),
                 ("kernel_launches_fp64", ctypes.c_int64),
+                ("timed_launches", ctypes.c_int64),
+                ("timed_updates", ctypes.c_int64),
                 ("iterations_run", ctypes.c_int32),
                 ("reserved", ctypes.c_int32),
+                ("timed_kernel_ms", ctypes.c_double),
+                ("last_iteration_change", ctypes.c_double),
                 ("last_solve_ms", ctypes.c_double),
                 ("sweep_ms_fp32", ctypes.c_double),
                 ("sweep_ms_fp64", ctypes.c_double)]
@@ -244,7 +250,7 @@ class SolverOptions:
                  algorithm=None, precision=Precision.FP64,
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
-                 use_graph=True, verbosity=0):
+                 use_graph=True, verbosity=0, kernel_timing=False):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -260,6 +266,7 @@ class SolverOptions:
         o.kernel = int(kernel)
         o.early_exit = 1 if early_exit else 0
         o.use_graph = 1 if use_graph else 0
+        o.kernel_timing = 1 if kernel_timing else 0
         o.verbosity = int(verbosity)
         self._o = o
 

@@ -256,6 +256,32 @@ HostOctant host_octant(const Problem3D &p, int sweep, bool initialize)
     return o;
 }
 
+// Updates a sweep makes in the columns that overlap the factored box (the
+// work of the factored column kernel; box_columns in ekx_kernels3d.hip).
+int64_t host_box_updates(const Problem3D &p, int sweep, bool initialize)
+{
+    if (p.radius < 0) { return 0; }
+    const int negX = sweep & 1, negY = (sweep >> 1) & 1;
+    int xLo = std::max(p.srcCellX - p.radius, 0);
+    int xHi = std::min(p.srcCellX + p.radius, p.nx - 1);
+    int yLo = std::max(p.srcCellY - p.radius, 0);
+    int yHi = std::min(p.srcCellY + p.radius, p.ny - 1);
+    if (xLo > xHi || yLo > yHi) { return 0; }
+    int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
+    int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
+    const HostOctant o = host_octant(p, sweep, initialize);
+    // nodes of columns bLo..bHi (8 wide) at or past j0, below n
+    auto span = [](int bLo, int bHi, int j0, int n)
+    {
+        int lo = std::max(bLo*kColumnTile, j0);
+        int hi = std::min(bHi*kColumnTile + kColumnTile - 1, n - 1);
+        return std::max(0, hi - lo + 1);
+    };
+    int64_t nxs = span(jxLo/kColumnTile, jxHi/kColumnTile, o.jx0, p.nx);
+    int64_t nys = span(jyLo/kColumnTile, jyHi/kColumnTile, o.jy0, p.ny);
+    return nxs*nys*(int64_t) (p.nz - o.jz0);
+}
+
 } // namespace
 
 namespace ekx
@@ -297,6 +323,14 @@ struct Engine3D
     hipStream_t sideStream = nullptr;   // factored-box columns
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     hipEvent_t evStart = nullptr, evStop = nullptr;
+    // kernel_timing: event pairs around the bulk column launches
+    std::vector<hipEvent_t> timingEvents;
+    size_t timingUsed = 0;
+    bool timing = false;
+    // early_exit: the largest decrease of the running iteration (float
+    // bits, atomic max), and its host copy
+    DeviceBuffer<unsigned int> change;
+    unsigned int *changeHost = nullptr;
     // segment events: [0,1] FP32 sweeps, [2,3] FP64 sweeps
     hipEvent_t evSeg[4] = {nullptr, nullptr, nullptr, nullptr};
     // The sweeps of one precision, captured once and replayed per solve.
@@ -315,7 +349,9 @@ struct Engine3D
             valid = false;
         }
     };
-    Segment segment[2];   // [0] primary sweeps, [1] FP64 polish sweeps
+    // [0] primary sweeps, [1] FP64 polish sweeps; early exit: [2] primary
+    // initialisation sweeps, [3] one primary refinement iteration
+    Segment segment[4];
     bool haveModel = false;
     bool haveField = false;
     bool fieldIsF64 = true;
@@ -328,6 +364,8 @@ struct Engine3D
         if (evStop) { (void) hipEventDestroy(evStop); }
         if (evFork) { (void) hipEventDestroy(evFork); }
         if (evJoin) { (void) hipEventDestroy(evJoin); }
+        for (hipEvent_t &e : timingEvents) { (void) hipEventDestroy(e); }
+        if (changeHost) { (void) hipHostFree(changeHost); }
         if (sideStream) { (void) hipStreamDestroy(sideStream); }
         for (hipEvent_t &e : evSeg)
         {
@@ -338,8 +376,12 @@ struct Engine3D
 
     void drop_graph()
     {
-        segment[0].release();
-        segment[1].release();
+        for (Segment &g : segment) { g.release(); }
+    }
+    bool early_exit() const
+    {
+        return opt.early_exit != 0 &&
+               opt.schedule == EKX_SCHEDULE_COMPLETED;
     }
 
     bool sweeps_f64() const
@@ -463,6 +505,17 @@ struct Engine3D
         else { T32.release(); }
         fieldIsF64 = sweeps_f64() || polish();
         EKX_CHECK(dProblems.allocate((size_t) n));
+        if (early_exit())
+        {
+            EKX_CHECK(change.allocate(1));
+            if (!changeHost)
+            {
+                EKX_HIP(hipHostMalloc((void **) &changeHost,
+                                      sizeof(unsigned int),
+                                      hipHostMallocDefault));
+            }
+        }
+        else { change.release(); }
         // Column work lists grouped by diagonal bx + by, all problems.
         std::vector<std::vector<ColumnItem>> perDiag;
         for (int i = 0; i < n; ++i)
@@ -635,6 +688,11 @@ struct Engine3D
                           (p.nz - o.jz0);
             st.updates += upd;
             if (sizeof(TS) == 8) { st.updates_fp64 += upd; }
+            if (kernel() == EKX_KERNEL_COLUMN)
+            {
+                st.updates_factored += host_box_updates(p, sweep,
+                                                        initialize);
+            }
         }
         int64_t launches = 0;
         ekx_status rc = enqueue_sweep_launches<TS>(sweep, initialize, T, S,
@@ -664,7 +722,7 @@ struct Engine3D
                     int nPairs = (kHi - kLo + 1)*(p.ny - o.jy0);
                     EKX_HIP(launch_hyperplane3d<TS>(
                         dProblems.ptr, i, sweep, initialize ? 1 : 0, L, kLo,
-                        nPairs, T, S, stream));
+                        nPairs, T, S, change.ptr, stream));
                     launches += 1;
                 }
             }
@@ -684,13 +742,16 @@ struct Engine3D
                     EKX_HIP(hipStreamWaitEvent(sideStream, evFork, 0));
                     EKX_HIP(launch_factored_columns3d<TS>(
                         dProblems.ptr, n, sweep, initialize ? 1 : 0, d,
-                        boxPerDiag, T, S, sideStream));
+                        boxPerDiag, T, S, change.ptr, sideStream));
                     launches += 1;
                 }
+                if (timing) { EKX_CHECK(record_timing_event()); }
                 EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
                                              dItems.ptr + diagStart[d], count,
                                              sweep, initialize ? 1 : 0, d,
-                                             boxPerDiag, T, S, stream));
+                                             boxPerDiag, T, S, change.ptr,
+                                             stream));
+                if (timing) { EKX_CHECK(record_timing_event()); }
                 launches += 1;
                 if (boxPerDiag > 0)
                 {
@@ -699,6 +760,20 @@ struct Engine3D
                 }
             }
         }
+        return EKX_OK;
+    }
+
+    // Records the next event of the timing pool on the main stream.
+    ekx_status record_timing_event()
+    {
+        if (timingUsed == timingEvents.size())
+        {
+            hipEvent_t e = nullptr;
+            EKX_HIP(hipEventCreate(&e));
+            timingEvents.push_back(e);
+        }
+        EKX_HIP(hipEventRecord(timingEvents[timingUsed], stream));
+        timingUsed += 1;
         return EKX_OK;
     }
 
@@ -741,7 +816,7 @@ struct Engine3D
                            int iterations, ekx_solve_stats &st)
     {
         ekx_solve_stats local{};
-        if (opt.use_graph)
+        if (opt.use_graph && !timing)
         {
             Segment &g = segment[k];
             if (!g.valid)
@@ -778,14 +853,53 @@ struct Engine3D
         }
         st.updates += local.updates;
         st.updates_fp64 += local.updates_fp64;
+        st.updates_factored += local.updates_factored;
         st.kernel_launches += local.kernel_launches;
         st.kernel_launches_fp64 += local.kernel_launches_fp64;
         st.iterations_run += local.iterations_run;
         return EKX_OK;
     }
 
+    // The sweeps of the first precision segment: one captured block, or,
+    // with early exit, the initialisation sweeps and then one refinement
+    // iteration at a time until an iteration's largest decrease is at most
+    // the tolerance.  kernel_timing times the bulk launches of this
+    // segment (issued eagerly).
+    template <typename TS>
+    ekx_status run_primary(TS *T, const TS *S, int iterations,
+                           ekx_solve_stats &st)
+    {
+        timing = opt.kernel_timing != 0;
+        timingUsed = 0;
+        if (!early_exit())
+        {
+            EKX_CHECK(run_segment<TS>(0, T, S, true, iterations, st));
+        }
+        else
+        {
+            EKX_CHECK(run_segment<TS>(2, T, S, true, 0, st));
+            for (int k = 0; k < iterations; ++k)
+            {
+                EKX_HIP(hipMemsetAsync(change.ptr, 0, sizeof(unsigned int),
+                                       stream));
+                EKX_CHECK(run_segment<TS>(3, T, S, false, 1, st));
+                EKX_HIP(hipMemcpyAsync(changeHost, change.ptr,
+                                       sizeof(unsigned int),
+                                       hipMemcpyDeviceToHost, stream));
+                EKX_HIP(hipStreamSynchronize(stream));
+                float c = 0;
+                std::memcpy(&c, changeHost, sizeof(float));
+                st.last_iteration_change = c;
+                if (c <= opt.convergence_tolerance) { break; }
+            }
+        }
+        timing = false;
+        return EKX_OK;
+    }
+
     ekx_status solve()
     {
+        timing = false;
         if (!haveModel)
         {
             return fail(EKX_ESTATE, "Velocity model not set");
@@ -800,8 +914,10 @@ struct Engine3D
         haveField = false;
         EKX_CHECK(prepare_sources());
         ekx_solve_stats st{};
+        st.last_iteration_change = -1;
         const int nIt = opt.number_of_sweeps;
         const int nProb = (int) problems.size();
+        const int itPrimary = polish() ? nIt - 1 : nIt;
         bool polished = false;
         EKX_HIP(hipEventRecord(evStart, stream));
         if (sweeps_f64())
@@ -809,48 +925,44 @@ struct Engine3D
             EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, maxNodes,
                                           T64.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
-            EKX_CHECK(run_segment<double>(0, T64.ptr, S64.ptr, true, nIt,
-                                          st));
+            EKX_CHECK(run_primary<double>(T64.ptr, S64.ptr, nIt, st));
             EKX_HIP(hipEventRecord(evSeg[1], stream));
         }
         else if (fixed32())
         {
-            int itQ = polish() ? nIt - 1 : nIt;
             EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, maxNodes,
                                             TQ.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
-            EKX_CHECK(run_segment<uint32_t>(0, TQ.ptr, SQ.ptr, true, itQ,
-                                            st));
+            EKX_CHECK(run_primary<uint32_t>(TQ.ptr, SQ.ptr, itPrimary, st));
             EKX_HIP(hipEventRecord(evSeg[1], stream));
-            if (polish())
-            {
-                EKX_HIP(launch_decode_times(dProblems.ptr, nProb, maxNodes,
-                                            TQ.ptr, T64.ptr, stream));
-                EKX_HIP(hipEventRecord(evSeg[2], stream));
-                EKX_CHECK(run_segment<double>(1, T64.ptr, S64.ptr, false, 1,
-                                              st));
-                EKX_HIP(hipEventRecord(evSeg[3], stream));
-                polished = true;
-            }
         }
         else
         {
-            int itF32 = polish() ? nIt - 1 : nIt;
             EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, maxNodes,
                                          T32.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
-            EKX_CHECK(run_segment<float>(0, T32.ptr, S32.ptr, true, itF32,
-                                         st));
+            EKX_CHECK(run_primary<float>(T32.ptr, S32.ptr, itPrimary, st));
             EKX_HIP(hipEventRecord(evSeg[1], stream));
-            if (polish())
+        }
+        // primary-segment work, for the timed-launch accounting
+        const int64_t primaryUpdates = st.updates;
+        const int64_t primaryFactored = st.updates_factored;
+        if (polish())
+        {
+            if (fixed32())
+            {
+                EKX_HIP(launch_decode_times(dProblems.ptr, nProb, maxNodes,
+                                            TQ.ptr, T64.ptr, stream));
+            }
+            else
             {
                 EKX_HIP(launch_convert(totalNodes, T32.ptr, T64.ptr, stream));
-                EKX_HIP(hipEventRecord(evSeg[2], stream));
-                EKX_CHECK(run_segment<double>(1, T64.ptr, S64.ptr, false, 1,
-                                              st));
-                EKX_HIP(hipEventRecord(evSeg[3], stream));
-                polished = true;
             }
+            EKX_HIP(hipEventRecord(evSeg[2], stream));
+            EKX_CHECK(run_segment<double>(1, T64.ptr, S64.ptr, false, 1,
+                                          st));
+            EKX_HIP(hipEventRecord(evSeg[3], stream));
+            polished = true;
         }
         EKX_HIP(hipEventRecord(evStop, stream));
         EKX_HIP(hipEventSynchronize(evStop));
@@ -865,6 +977,19 @@ struct Engine3D
         {
             EKX_HIP(hipEventElapsedTime(&ms, evSeg[2], evSeg[3]));
             st.sweep_ms_fp64 = ms;
+        }
+        if (timingUsed > 0)
+        {
+            double total = 0;
+            for (size_t i = 0; i + 1 < timingUsed; i += 2)
+            {
+                EKX_HIP(hipEventElapsedTime(&ms, timingEvents[i],
+                                            timingEvents[i + 1]));
+                total += ms;
+            }
+            st.timed_kernel_ms = total;
+            st.timed_launches = (int64_t) (timingUsed/2);
+            st.timed_updates = primaryUpdates - primaryFactored;
         }
         stats = st;
         haveField = true;
@@ -949,6 +1074,7 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->kernel = EKX_KERNEL_AUTO;
     o->early_exit = 0;
     o->use_graph = 1;
+    o->kernel_timing = 0;
 }
 
 ekx_status ekx_device_count(int *count)

@@ -21,14 +21,15 @@ template <typename TS>
 hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,
                                int nPairs, TS *T, const TS *S,
-                               hipStream_t stream);
+                               unsigned int *change, hipStream_t stream);
 // One column diagonal, Cartesian operators: the diagonal's items, except
 // the columns overlapping a factored box (boxPerDiag > 0).
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
-                            TS *T, const TS *S, hipStream_t stream);
+                            TS *T, const TS *S, unsigned int *change,
+                            hipStream_t stream);
 // The same diagonal's columns that overlap a factored box: up to
 // boxPerDiag per problem, full local update.  Independent of the launch
 // above (disjoint columns), so the two may run concurrently.
@@ -36,7 +37,11 @@ template <typename TS>
 hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      int nProblems, int sweep, int initialize,
                                      int diagonal, int boxPerDiag, TS *T,
-                                     const TS *S, hipStream_t stream);
+                                     const TS *S, unsigned int *change,
+                                     hipStream_t stream);
+// Sweep launches take `change`: NULL, or a device word that receives (as
+// float bits, by atomic max) the largest decrease any update of the launch
+// made -- the convergence monitor of the early-exit option.
 hipError_t launch_nodal_to_cell3d(int nx, int ny, int nz, const double *v,
                                   int ordering, double *slow,
                                   hipStream_t stream);

@@ -109,7 +109,8 @@ template <typename TS>
 __global__ void __launch_bounds__(256)
 sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
                    int sweep, int initialize, int level, int kLo, int nPairs,
-                   TS *__restrict__ T, const TS *__restrict__ S)
+                   TS *__restrict__ T, const TS *__restrict__ S,
+                   unsigned int *__restrict__ change)
 {
     const int pid = blockIdx.x*blockDim.x + threadIdx.x;
     if (pid >= nPairs) { return; }
@@ -169,7 +170,18 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
     const R tUpd = local_update3d<R>(p, sw, ix, iy, iz, s0, s1, s2, s3, s4,
                                      s5, s7, t1, t2, t3, t4, t5, t6, t7);
     const TS c = St::code(tUpd, p);
-    if (c < own) { Tp[i0] = c; }
+    if (c < own)
+    {
+        Tp[i0] = c;
+        if (change != nullptr)
+        {
+            // largest decrease of the sweep (positive floats order as
+            // their bit patterns)
+            const float drop = (float) (St::time(own, p) -
+                                        St::finite_time(c, p));
+            atomicMax(change, __float_as_uint(drop));
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -203,7 +215,8 @@ __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
                 const ColumnItem *__restrict__ items, int nItems, int sweep,
                 int initialize, int diagonal, int boxPerDiag,
-                TS *__restrict__ T, const TS *__restrict__ S)
+                TS *__restrict__ T, const TS *__restrict__ S,
+                unsigned int *__restrict__ change)
 {
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
     // kColumnRing slots plus one spare that absorbs the stores of the
@@ -342,6 +355,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // z-neighbours (plane q - 1) of the lane's node, decoded
     R pt4 = (R) kHuge, pt5 = (R) kHuge, pt6 = (R) kHuge, pt7 = (R) kHuge;
     R ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
+    R maxDrop = 0;                        // largest decrease by this lane
     // The node is local (a+1, b+1).  Its neighbours (x-1, y-1, xy-1) and
     // its 4 cells of one plane share these four LDS offsets.
     const int oA = lidx(a, b), oB = lidx(a + 1, b), oC = lidx(a, b + 1);
@@ -426,7 +440,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                     const TS c = St::code(tUpd, p);
                     if (c < ownCode)
                     {
-                        tOwn = St::finite_time(c, p);
+                        const R tNew = St::finite_time(c, p);
+                        maxDrop = Math<R>::max(maxDrop, tOwn - tNew);
+                        tOwn = tNew;
                         sT[oD] = c;
                     }
                 }
@@ -453,6 +469,22 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // step's: other lanes of the wave consume it.
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // Convergence monitor: the wave's largest decrease, one atomic per
+    // column.  Every lane of the wave is still active here (the early
+    // returns above are whole-wave).
+    if (change != nullptr)
+    {
+        float d = (float) maxDrop;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            d = fmaxf(d, __shfl_xor(d, off, 64));
+        }
+        if (lane == 0 && d > 0.0f)
+        {
+            atomicMax(change, __float_as_uint(d));
         }
     }
 }
@@ -618,13 +650,13 @@ template <typename TS>
 hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,
                                int nPairs, TS *T, const TS *S,
-                               hipStream_t stream)
+                               unsigned int *change, hipStream_t stream)
 {
     if (nPairs <= 0) { return hipSuccess; }
     hipLaunchKernelGGL(sweep_hyperplane3d<TS>,
                        dim3((unsigned) ((nPairs + 255)/256)), dim3(256), 0,
                        stream, dProblems, problem, sweep, initialize, level,
-                       kLo, nPairs, T, S);
+                       kLo, nPairs, T, S, change);
     return hipGetLastError();
 }
 
@@ -632,14 +664,15 @@ template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
-                            TS *T, const TS *S, hipStream_t stream)
+                            TS *T, const TS *S, unsigned int *change,
+                            hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
     constexpr int W = column_waves<TS>();
     hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>),
                        dim3((unsigned) ((nItems + W - 1)/W)), dim3(64*W), 0,
                        stream, dProblems, dItems, nItems, sweep, initialize,
-                       diagonal, boxPerDiag, T, S);
+                       diagonal, boxPerDiag, T, S, change);
     return hipGetLastError();
 }
 
@@ -647,14 +680,15 @@ template <typename TS>
 hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      int nProblems, int sweep, int initialize,
                                      int diagonal, int boxPerDiag, TS *T,
-                                     const TS *S, hipStream_t stream)
+                                     const TS *S, unsigned int *change,
+                                     hipStream_t stream)
 {
     if (boxPerDiag <= 0) { return hipSuccess; }
     int n = nProblems*boxPerDiag;
     hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true>), dim3((unsigned) n),
                        dim3(64), 0, stream, dProblems,
                        (const ColumnItem *) nullptr, n, sweep, initialize,
-                       diagonal, boxPerDiag, T, S);
+                       diagonal, boxPerDiag, T, S, change);
     return hipGetLastError();
 }
 
@@ -730,42 +764,42 @@ template hipError_t launch_init3d<double>(const Problem3D *, int, int64_t,
                                           double *, hipStream_t);
 template hipError_t launch_hyperplane3d<float>(const Problem3D *, int, int,
                                                int, int, int, int, float *,
-                                               const float *, hipStream_t);
+                                               const float *, unsigned int *, hipStream_t);
 template hipError_t launch_hyperplane3d<double>(const Problem3D *, int, int,
                                                 int, int, int, int, double *,
-                                                const double *, hipStream_t);
+                                                const double *, unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<float>(const Problem3D *,
                                             const ColumnItem *, int, int, int,
                                             int, int, float *, const float *,
-                                            hipStream_t);
+                                            unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<double>(const Problem3D *,
                                              const ColumnItem *, int, int,
                                              int, int, int, double *,
-                                             const double *, hipStream_t);
+                                             const double *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<float>(const Problem3D *, int,
                                                      int, int, int, int,
                                                      float *, const float *,
-                                                     hipStream_t);
+                                                     unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<double>(const Problem3D *, int,
                                                       int, int, int, int,
                                                       double *,
                                                       const double *,
-                                                      hipStream_t);
+                                                      unsigned int *, hipStream_t);
 template hipError_t launch_init3d<uint32_t>(const Problem3D *, int, int64_t,
                                             uint32_t *, hipStream_t);
 template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
                                                   int, int, int, int, int,
                                                   uint32_t *,
                                                   const uint32_t *,
-                                                  hipStream_t);
+                                                  unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<uint32_t>(const Problem3D *,
                                                const ColumnItem *, int, int,
                                                int, int, int, uint32_t *,
-                                               const uint32_t *, hipStream_t);
+                                               const uint32_t *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<uint32_t>(const Problem3D *,
                                                         int, int, int, int,
                                                         int, uint32_t *,
                                                         const uint32_t *,
-                                                        hipStream_t);
+                                                        unsigned int *, hipStream_t);
 
 } // namespace ekx

@@ -100,12 +100,18 @@ typedef struct
     int schedule;                  /* ekx_schedule, default COMPLETED      */
     int kernel;                    /* ekx_kernel, default AUTO             */
     int early_exit;                /* 0 (default): always run every
-                                      iteration, as the reference does;
-                                      1: stop refinement iterations once the
-                                      largest change is below the
-                                      tolerance (FP32 modes only)          */
+                                      iteration, as the reference does (its
+                                      tolerance is never tested);
+                                      1: stop the refinement iterations once
+                                      no update of an iteration lowered a
+                                      travel time by more than
+                                      convergence_tolerance (3D only)       */
     int use_graph;                 /* 1 (default): replay solves from a
                                       captured hipGraph                     */
+    int kernel_timing;             /* 1: issue the launches eagerly and time
+                                      every bulk column-kernel launch of the
+                                      first precision segment with HIP
+                                      events (ekx_solve_stats.timed_*)      */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */
@@ -130,10 +136,19 @@ typedef struct
     int64_t updates;              /* local-solver evaluations: every node
                                      inside a sweep's update range        */
     int64_t updates_fp64;         /* of which evaluated in FP64 sweeps     */
+    int64_t updates_factored;     /* of which by the factored-box column
+                                     kernel (column kernel only)          */
     int64_t kernel_launches;      /* sweep kernel launches                 */
     int64_t kernel_launches_fp64; /* of which FP64 sweep launches          */
+    int64_t timed_launches;       /* bulk column launches timed
+                                     (kernel_timing)                       */
+    int64_t timed_updates;        /* updates done by the timed launches    */
     int32_t iterations_run;       /* refinement iterations actually run    */
     int32_t reserved;
+    double timed_kernel_ms;       /* summed durations of the timed launches */
+    double last_iteration_change; /* early_exit: largest decrease of one
+                                     update in the last refinement
+                                     iteration (s); -1 otherwise           */
     double last_solve_ms;         /* device time of the whole last solve   */
     double sweep_ms_fp32;         /* device time of the sweep launches on
                                      4-byte storage (FP32 or FIXED32)      */

@@ -1,0 +1,91 @@
+"""Solver options beyond the reference's: early exit on convergence, eager
+launches with per-launch kernel timing, graph replay on/off.  Each must
+leave the arithmetic, and so the FP64 field, unchanged."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def layered():
+    from eikonalxx_amd import workloads
+    w = workloads.config3_mixed(nx=65, ny=57, nz=49, h=25.0)
+    slow = workloads.cell_slowness(w["velocity"], w["nx"], w["ny"], w["nz"])
+    return w, slow
+
+
+def _solve(ekx, w, slow, **kw):
+    s = ekx.Solver3D()
+    s.initialize(ekx.SolverOptions(factored_radius=w["radius"], **kw),
+                 ekx.Geometry3D(w["nx"], w["ny"], w["nz"], w["h"]))
+    s.set_slowness(slow)
+    s.set_source(w["source"])
+    s.solve()
+    return s.get_travel_time_field(), s.get_stats()
+
+
+def test_graph_and_eager_agree(ekx, layered):
+    w, slow = layered
+    a, _ = _solve(ekx, w, slow, number_of_sweeps=2, use_graph=True)
+    b, _ = _solve(ekx, w, slow, number_of_sweeps=2, use_graph=False)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["FP64", "FIXED32", "FP32_POLISH"])
+def test_kernel_timing(ekx, layered, prec):
+    """Eager launches timed one by one: same field as the graph replay,
+    and the timed kernels account for the bulk of the segment."""
+    w, slow = layered
+    p = getattr(ekx.Precision, prec)
+    a, _ = _solve(ekx, w, slow, number_of_sweeps=2, precision=p)
+    b, st = _solve(ekx, w, slow, number_of_sweeps=2, precision=p,
+                   kernel_timing=True)
+    assert np.array_equal(a, b)
+    seg_ms = st.sweep_ms_fp64 if prec == "FP64" else st.sweep_ms_fp32
+    print(prec, st.as_dict())
+    assert st.timed_launches > 0
+    assert 0 < st.timed_kernel_ms <= seg_ms * 1.01
+    assert 0 < st.updates_factored < st.timed_updates < st.updates
+
+
+def test_early_exit_zero_tolerance_runs_every_iteration(ekx, layered):
+    w, slow = layered
+    a, _ = _solve(ekx, w, slow, number_of_sweeps=3)
+    b, st = _solve(ekx, w, slow, number_of_sweeps=3, early_exit=True,
+                   convergence_tolerance=0.0)
+    assert np.array_equal(a, b)
+    assert st.iterations_run == 3
+    assert st.last_iteration_change >= 0.0
+
+
+def test_early_exit_stops_when_converged(ekx, layered):
+    """A loose tolerance stops after the first iteration whose largest
+    decrease is below it; the field is then exactly that of a solve with
+    that many iterations."""
+    w, slow = layered
+    _, st0 = _solve(ekx, w, slow, number_of_sweeps=1, early_exit=True,
+                    convergence_tolerance=0.0)
+    tol = st0.last_iteration_change * 1.5 + 1e-12
+    t, st = _solve(ekx, w, slow, number_of_sweeps=6, early_exit=True,
+                   convergence_tolerance=tol)
+    assert 1 <= st.iterations_run < 6
+    assert st.last_iteration_change <= tol
+    ref, _ = _solve(ekx, w, slow, number_of_sweeps=st.iterations_run)
+    assert np.array_equal(t, ref)
+    # a tolerance above any change stops after one iteration
+    t1, st1 = _solve(ekx, w, slow, number_of_sweeps=6, early_exit=True,
+                     convergence_tolerance=100.0)
+    assert st1.iterations_run == 1
+    ref1, _ = _solve(ekx, w, slow, number_of_sweeps=1)
+    assert np.array_equal(t1, ref1)
+
+
+def test_early_exit_monitor_on_hyperplane_kernel(ekx, layered):
+    w, slow = layered
+    a, sa = _solve(ekx, w, slow, number_of_sweeps=2, early_exit=True,
+                   convergence_tolerance=0.0)
+    b, sb = _solve(ekx, w, slow, number_of_sweeps=2, early_exit=True,
+                   convergence_tolerance=0.0, kernel=ekx.Kernel.HYPERPLANE)
+    assert np.array_equal(a, b)
+    assert sa.last_iteration_change == sb.last_iteration_change
