@@ -18,12 +18,17 @@
  *   source location -> cell     -> testing/source.cpp closed forms.
  *   nodal -> cell slowness      -> testing/model.cpp closed forms.
  *   analytic fields             -> closed forms (testing/analytic.cpp).
- *   3D/2D local solvers         -> PARTIAL: plane-wave exactness of the 1D,
- *       2D and 8-point operators, the factored operators returning T0 for
- *       T0 neighbours in homogeneous media, monotonicity/causality
- *       invariants, and convergence to the analytic fields.  Bitwise parity
- *       with the reference binary itself is unpinned: the reference needs
- *       SYCL and TBB headers that this image lacks.
+ *   3D/2D Cartesian operators   -> plane-wave exactness of the 1D, 2D and
+ *       8-point operators (axis and diagonal fronts, every sweep); the
+ *       non-uniform 3D restatement against the uniform one on equal
+ *       spacing; sweeps never raise a time; serial == level-set order.
+ *   3D/2D factored operators    -> PARTIAL ("parity unpinned" beyond this):
+ *       no closed-form special case exists (the reference quadratics do not
+ *       return T0 on exact T0 data), so only first-order convergence of
+ *       whole solves to the analytic fields and the line-by-line
+ *       restatement pin them.
+ *   Bitwise parity with the reference binary itself is unpinned: the
+ *   reference needs SYCL and TBB headers that this image lacks.
  */
 #ifndef EKX_ORACLE_H
 #define EKX_ORACLE_H
