@@ -118,7 +118,9 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
     arith, nbytes, kernel, text = MODES[mode]
     opts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
                              factored_radius=w["radius"],
-                             precision=getattr(ekx.Precision, mode))
+                             precision=getattr(ekx.Precision, mode),
+                             chains=args.chains,
+                             prefetch_planes=args.prefetch)
     batch = ekx.Batch3D()
     batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
     for i in range(args.warmup):
@@ -169,7 +171,8 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
         topts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
                                   factored_radius=w["radius"],
                                   precision=getattr(ekx.Precision, mode),
-                                  kernel_timing=True)
+                                  kernel_timing=True, chains=args.chains,
+                                  prefetch_planes=args.prefetch)
         tb = ekx.Batch3D()
         tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
         tb.solve()
@@ -254,6 +257,12 @@ def main():
                     help="comma-separated precision modes to time; the "
                          "headline is the fastest within the error bar")
     ap.add_argument("--check-sources", type=int, default=8)
+    ap.add_argument("--prefetch", type=int, default=0,
+                    help="column-kernel planes prefetched ahead (4, 8, 12; "
+                         "0: the library's choice)")
+    ap.add_argument("--chains", type=int, default=0,
+                    help="independent launch chains (0: the library's "
+                         "choice)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-iterations", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")

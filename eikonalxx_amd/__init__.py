@@ -18,7 +18,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "libeikonalxx_amd.so")
+# EKX_LIBRARY_PATH selects another build of the same library (for
+# comparing builds side by side); the default is the in-tree build.
+LIBRARY_PATH = os.environ.get("EKX_LIBRARY_PATH",
+                              os.path.join(_HERE, "libeikonalxx_amd.so"))
 
 __all__ = ["SolverAlgorithm", "Precision", "Schedule", "Kernel", "Ordering",
            "SolverOptions", "Geometry3D", "Geometry2D", "Solver3D",
@@ -73,7 +76,9 @@ class _Options(ctypes.Structure):
                 ("kernel", ctypes.c_int),
                 ("early_exit", ctypes.c_int),
                 ("use_graph", ctypes.c_int),
-                ("kernel_timing", ctypes.c_int)]
+                ("kernel_timing", ctypes.c_int),
+                ("chains", ctypes.c_int),
+                ("prefetch_planes", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -250,7 +255,8 @@ class SolverOptions:
                  algorithm=None, precision=Precision.FP64,
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
-                 use_graph=True, verbosity=0, kernel_timing=False):
+                 use_graph=True, verbosity=0, kernel_timing=False,
+                 chains=0, prefetch_planes=0):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -267,6 +273,8 @@ class SolverOptions:
         o.early_exit = 1 if early_exit else 0
         o.use_graph = 1 if use_graph else 0
         o.kernel_timing = 1 if kernel_timing else 0
+        o.chains = int(chains)
+        o.prefetch_planes = int(prefetch_planes)
         o.verbosity = int(verbosity)
         self._o = o
 

@@ -22,6 +22,8 @@ namespace
 
 // Blocks of the model-maximum reduction (FIXED32 modes).
 constexpr size_t kMaxPartials = 1024;
+// Independent launch chains of a batch (one stream each).
+constexpr int kMaxChains = 16;
 
 
 thread_local std::string g_error;
@@ -149,6 +151,16 @@ ekx_status check_options(const ekx_solver_options *o)
         o->schedule != EKX_SCHEDULE_SHIPPED)
     {
         return fail(EKX_EINVAL, "unknown schedule");
+    }
+    if (o->chains < 0 || o->chains > kMaxChains)
+    {
+        return fail(EKX_EINVAL, "chains must be in [0, " +
+                                    std::to_string(kMaxChains) + "]");
+    }
+    if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
+        o->prefetch_planes != 8 && o->prefetch_planes != 12)
+    {
+        return fail(EKX_EINVAL, "prefetch_planes must be 0, 4, 8 or 12");
     }
     if (o->kernel < EKX_KERNEL_AUTO || o->kernel > EKX_KERNEL_COLUMN)
     {
@@ -314,7 +326,15 @@ struct Engine3D
     std::vector<double> modelSMax;   // largest slowness of each model
     DeviceBuffer<Problem3D> dProblems;
     DeviceBuffer<ColumnItem> dItems;
+    // Column items ordered by chain, then diagonal bx + by: chain c's items
+    // of diagonal d are [diagStart[c*(nDiag+1)+d], ...[+d+1]).  A chain is
+    // a contiguous range of problems [chainBegin[c], chainBegin[c+1]).
     std::vector<int> diagStart;
+    int nDiag = 0;
+    int nChains = 1;
+    std::vector<int> chainBegin;
+    std::vector<hipStream_t> chainStreams;
+    std::vector<hipEvent_t> chainEvents;
     // Upper bound, over any source position, of the columns per diagonal
     // that overlap one problem's factored box: the box spans at most
     // ceil((2r+1)/8)+1 columns per axis.
@@ -365,6 +385,8 @@ struct Engine3D
         if (evFork) { (void) hipEventDestroy(evFork); }
         if (evJoin) { (void) hipEventDestroy(evJoin); }
         for (hipEvent_t &e : timingEvents) { (void) hipEventDestroy(e); }
+        for (hipEvent_t &e : chainEvents) { (void) hipEventDestroy(e); }
+        for (hipStream_t &c : chainStreams) { (void) hipStreamDestroy(c); }
         if (changeHost) { (void) hipHostFree(changeHost); }
         if (sideStream) { (void) hipStreamDestroy(sideStream); }
         for (hipEvent_t &e : evSeg)
@@ -399,6 +421,10 @@ struct Engine3D
                 opt.precision == EKX_PRECISION_FIXED32_POLISH) &&
                opt.schedule == EKX_SCHEDULE_COMPLETED &&
                opt.number_of_sweeps >= 1;
+    }
+    int depth() const
+    {
+        return opt.prefetch_planes > 0 ? opt.prefetch_planes : 4;
     }
     int kernel() const
     {
@@ -516,24 +542,71 @@ struct Engine3D
             }
         }
         else { change.release(); }
-        // Column work lists grouped by diagonal bx + by, all problems.
-        std::vector<std::vector<ColumnItem>> perDiag;
+        // Chains: contiguous ranges of problems with about equal work
+        // (node count).  One chain unless the batch is large enough to
+        // keep several independent launch sequences busy.
+        nChains = o.chains > 0 ? std::min(o.chains, n)
+                               : (n >= 16 ? std::min(4, n) : 1);
+        if (kernel() != EKX_KERNEL_COLUMN) { nChains = 1; }
+        chainBegin.assign(1, 0);
+        {
+            int64_t done = 0;
+            for (int i = 0; i < n; ++i)
+            {
+                const ekx_geometry3d &g = geoms[i];
+                done += (int64_t) g.nx*g.ny*g.nz;
+                int c = (int) chainBegin.size();      // chain being filled
+                if (c < nChains &&
+                    done*nChains >= totalNodes*(int64_t) c && i + 1 < n)
+                {
+                    chainBegin.push_back(i + 1);
+                }
+            }
+            chainBegin.push_back(n);
+            nChains = (int) chainBegin.size() - 1;
+        }
+        while ((int) chainStreams.size() < nChains)
+        {
+            hipStream_t cs = nullptr;
+            hipEvent_t ce = nullptr;
+            EKX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            EKX_HIP(hipEventCreateWithFlags(&ce, hipEventDisableTiming));
+            chainStreams.push_back(cs);
+            chainEvents.push_back(ce);
+        }
+        // Column work lists by chain, then diagonal bx + by.
+        nDiag = 0;
         for (int i = 0; i < n; ++i)
         {
             const ekx_geometry3d &g = geoms[i];
             int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
             int nby = (g.ny + kColumnTile - 1)/kColumnTile;
-            if ((int) perDiag.size() < nbx + nby - 1)
+            nDiag = std::max(nDiag, nbx + nby - 1);
+        }
+        std::vector<ColumnItem> flat;
+        diagStart.clear();
+        for (int c = 0; c < nChains; ++c)
+        {
+            std::vector<std::vector<ColumnItem>> perDiag(nDiag);
+            for (int i = chainBegin[c]; i < chainBegin[c + 1]; ++i)
             {
-                perDiag.resize(nbx + nby - 1);
-            }
-            for (int by = 0; by < nby; ++by)
-            {
-                for (int bx = 0; bx < nbx; ++bx)
+                const ekx_geometry3d &g = geoms[i];
+                int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
+                int nby = (g.ny + kColumnTile - 1)/kColumnTile;
+                for (int by = 0; by < nby; ++by)
                 {
-                    perDiag[bx + by].push_back(ColumnItem{i, bx, by, 0});
+                    for (int bx = 0; bx < nbx; ++bx)
+                    {
+                        perDiag[bx + by].push_back(ColumnItem{i, bx, by, 0});
+                    }
                 }
             }
+            for (auto &d : perDiag)
+            {
+                diagStart.push_back((int) flat.size());
+                flat.insert(flat.end(), d.begin(), d.end());
+            }
+            diagStart.push_back((int) flat.size());
         }
         boxPerDiag = 0;
         if (o.factored_radius >= 0)
@@ -548,13 +621,6 @@ struct Engine3D
                 boxPerDiag = std::max(boxPerDiag,
                                       std::min(span, std::min(nbx, nby)));
             }
-        }
-        std::vector<ColumnItem> flat;
-        diagStart.assign(1, 0);
-        for (auto &d : perDiag)
-        {
-            flat.insert(flat.end(), d.begin(), d.end());
-            diagStart.push_back((int) flat.size());
         }
         EKX_CHECK(dItems.allocate(flat.size()));
         EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
@@ -675,14 +741,12 @@ struct Engine3D
         p.sStep = sMax/(double) kTimeMaxCode;
     }
 
+    // Work counters of one sweep over every problem.
     template <typename TS>
-    ekx_status enqueue_sweep(int sweep, bool initialize, TS *T, const TS *S,
-                             ekx_solve_stats &st)
+    void count_sweep(int sweep, bool initialize, ekx_solve_stats &st) const
     {
-        const int n = (int) problems.size();
-        for (int i = 0; i < n; ++i)
+        for (const Problem3D &p : problems)
         {
-            const Problem3D &p = problems[i];
             HostOctant o = host_octant(p, sweep, initialize);
             int64_t upd = (int64_t) (p.nx - o.jx0)*(p.ny - o.jy0)*
                           (p.nz - o.jz0);
@@ -694,12 +758,54 @@ struct Engine3D
                                                         initialize);
             }
         }
+    }
+
+    template <typename TS>
+    ekx_status enqueue_sweep(int sweep, bool initialize, TS *T, const TS *S,
+                             ekx_solve_stats &st)
+    {
+        count_sweep<TS>(sweep, initialize, st);
         int64_t launches = 0;
         ekx_status rc = enqueue_sweep_launches<TS>(sweep, initialize, T, S,
                                                    launches);
         st.kernel_launches += launches;
         if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
         return rc;
+    }
+
+    // One sweep of chain c on its own stream: per diagonal, the bulk
+    // launch over the chain's items, then the factored-box columns of the
+    // chain's problems.
+    template <typename TS>
+    ekx_status enqueue_chain_sweep(int c, int sweep, bool initialize, TS *T,
+                                   const TS *S, int64_t &launches)
+    {
+        hipStream_t cs = chainStreams[c];
+        const int p0 = chainBegin[c];
+        const int np = chainBegin[c + 1] - p0;
+        const int *ds = diagStart.data() + (size_t) c*(nDiag + 1);
+        for (int d = 0; d < nDiag; ++d)
+        {
+            const int count = ds[d + 1] - ds[d];
+            if (count > 0)
+            {
+                if (timing) { EKX_CHECK(record_timing_event(cs)); }
+                EKX_HIP(launch_columns3d<TS>(dProblems.ptr, dItems.ptr + ds[d],
+                                             count, sweep, initialize ? 1 : 0,
+                                             d, boxPerDiag, T, S, change.ptr,
+                                             depth(), cs));
+                if (timing) { EKX_CHECK(record_timing_event(cs)); }
+                launches += 1;
+            }
+            if (boxPerDiag > 0)
+            {
+                EKX_HIP(launch_factored_columns3d<TS>(
+                    dProblems.ptr + p0, np, sweep, initialize ? 1 : 0, d,
+                    boxPerDiag, T, S, change.ptr, cs));
+                launches += 1;
+            }
+        }
+        return EKX_OK;
     }
 
     template <typename TS>
@@ -732,7 +838,6 @@ struct Engine3D
             // Per diagonal: the factored-box columns run on the side stream
             // concurrently with the bulk (Cartesian) launch; both join
             // before the next diagonal.
-            const int nDiag = (int) diagStart.size() - 1;
             for (int d = 0; d < nDiag; ++d)
             {
                 int count = diagStart[d + 1] - diagStart[d];
@@ -745,13 +850,13 @@ struct Engine3D
                         boxPerDiag, T, S, change.ptr, sideStream));
                     launches += 1;
                 }
-                if (timing) { EKX_CHECK(record_timing_event()); }
+                if (timing) { EKX_CHECK(record_timing_event(stream)); }
                 EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
                                              dItems.ptr + diagStart[d], count,
                                              sweep, initialize ? 1 : 0, d,
                                              boxPerDiag, T, S, change.ptr,
-                                             stream));
-                if (timing) { EKX_CHECK(record_timing_event()); }
+                                             depth(), stream));
+                if (timing) { EKX_CHECK(record_timing_event(stream)); }
                 launches += 1;
                 if (boxPerDiag > 0)
                 {
@@ -763,8 +868,8 @@ struct Engine3D
         return EKX_OK;
     }
 
-    // Records the next event of the timing pool on the main stream.
-    ekx_status record_timing_event()
+    // Records the next event of the timing pool on stream s.
+    ekx_status record_timing_event(hipStream_t s)
     {
         if (timingUsed == timingEvents.size())
         {
@@ -772,7 +877,7 @@ struct Engine3D
             EKX_HIP(hipEventCreate(&e));
             timingEvents.push_back(e);
         }
-        EKX_HIP(hipEventRecord(timingEvents[timingUsed], stream));
+        EKX_HIP(hipEventRecord(timingEvents[timingUsed], s));
         timingUsed += 1;
         return EKX_OK;
     }
@@ -781,6 +886,7 @@ struct Engine3D
     ekx_status enqueue_phase_sweeps(TS *T, const TS *S, bool doInit,
                                     int iterations, ekx_solve_stats &st)
     {
+        std::vector<std::pair<int, bool>> list;
         if (doInit)
         {
             if (opt.schedule == EKX_SCHEDULE_SHIPPED)
@@ -788,24 +894,54 @@ struct Engine3D
                 // src/solver3d.cpp:451-463 (FSM); level set: nothing.
                 if (opt.algorithm == EKX_FAST_SWEEPING_METHOD)
                 {
-                    EKX_CHECK(enqueue_sweep<TS>(0, true, T, S, st));
-                    EKX_CHECK(enqueue_sweep<TS>(1, true, T, S, st));
+                    list = {{0, true}, {1, true}};
                 }
-                return EKX_OK;
+                iterations = 0;
             }
-            for (int s = 0; s < 8; ++s)
+            else
             {
-                EKX_CHECK(enqueue_sweep<TS>(s, true, T, S, st));
+                for (int s = 0; s < 8; ++s) { list.push_back({s, true}); }
             }
         }
         for (int k = 0; k < iterations; ++k)
         {
-            for (int s = 0; s < 8; ++s)
-            {
-                EKX_CHECK(enqueue_sweep<TS>(s, false, T, S, st));
-            }
-            st.iterations_run += 1;
+            for (int s = 0; s < 8; ++s) { list.push_back({s, false}); }
         }
+        if (nChains == 1)
+        {
+            for (const auto &sw : list)
+            {
+                EKX_CHECK(enqueue_sweep<TS>(sw.first, sw.second, T, S, st));
+            }
+        }
+        else
+        {
+            // Chains run their whole sweep list independently: fork from
+            // the main stream, join back at the end.
+            for (const auto &sw : list)
+            {
+                count_sweep<TS>(sw.first, sw.second, st);
+            }
+            int64_t launches = 0;
+            EKX_HIP(hipEventRecord(evFork, stream));
+            for (int c = 0; c < nChains; ++c)
+            {
+                EKX_HIP(hipStreamWaitEvent(chainStreams[c], evFork, 0));
+                for (const auto &sw : list)
+                {
+                    EKX_CHECK(enqueue_chain_sweep<TS>(c, sw.first, sw.second,
+                                                      T, S, launches));
+                }
+                EKX_HIP(hipEventRecord(chainEvents[c], chainStreams[c]));
+            }
+            for (int c = 0; c < nChains; ++c)
+            {
+                EKX_HIP(hipStreamWaitEvent(stream, chainEvents[c], 0));
+            }
+            st.kernel_launches += launches;
+            if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
+        }
+        st.iterations_run += iterations;
         return EKX_OK;
     }
 
@@ -1075,6 +1211,8 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->early_exit = 0;
     o->use_graph = 1;
     o->kernel_timing = 0;
+    o->chains = 0;
+    o->prefetch_planes = 0;
 }
 
 ekx_status ekx_device_count(int *count)

@@ -23,13 +23,14 @@ hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int nPairs, TS *T, const TS *S,
                                unsigned int *change, hipStream_t stream);
 // One column diagonal, Cartesian operators: the diagonal's items, except
-// the columns overlapping a factored box (boxPerDiag > 0).
+// the columns overlapping a factored box (boxPerDiag > 0).  depth: planes
+// prefetched ahead of the compute (4, 8 or 12).
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
                             TS *T, const TS *S, unsigned int *change,
-                            hipStream_t stream);
+                            int depth, hipStream_t stream);
 // The same diagonal's columns that overlap a factored box: up to
 // boxPerDiag per problem, full local update.  Independent of the launch
 // above (disjoint columns), so the two may run concurrently.

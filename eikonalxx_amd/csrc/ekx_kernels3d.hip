@@ -247,7 +247,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         by = item.by;
     }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
-    const Problem3D &p = problems[pIndex];
+    // A private copy: the fields the loop uses are loaded once, into
+    // scalar registers, instead of being re-read every step.
+    const Problem3D p = problems[pIndex];
     if (!FACTORED)
     {
         const BoxColumns bc = box_columns(p, sweep);
@@ -466,8 +468,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 Tp[(int64_t) az*nxy + wbXY] = wbValue;
             }
             // LDS traffic of this step must not be reordered with the next
-            // step's: other lanes of the wave consume it.
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            // step's: other lanes of the wave consume it.  The fence covers
+            // LDS only, so global loads stay free to move.
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
             __builtin_amdgcn_wave_barrier();
         }
     }
@@ -665,14 +668,31 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
                             TS *T, const TS *S, unsigned int *change,
-                            hipStream_t stream)
+                            int depth, hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
     constexpr int W = column_waves<TS>();
-    hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>),
-                       dim3((unsigned) ((nItems + W - 1)/W)), dim3(64*W), 0,
-                       stream, dProblems, dItems, nItems, sweep, initialize,
-                       diagonal, boxPerDiag, T, S, change);
+    const dim3 grid((unsigned) ((nItems + W - 1)/W)), block(64*W);
+    switch (depth)
+    {
+    case 4:
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>), grid, block,
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, T, S, change);
+        break;
+    case 8:
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false>), grid, block,
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, T, S, change);
+        break;
+    case 12:
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false>), grid, block,
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, T, S, change);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -771,11 +791,11 @@ template hipError_t launch_hyperplane3d<double>(const Problem3D *, int, int,
 template hipError_t launch_columns3d<float>(const Problem3D *,
                                             const ColumnItem *, int, int, int,
                                             int, int, float *, const float *,
-                                            unsigned int *, hipStream_t);
+                                            unsigned int *, int, hipStream_t);
 template hipError_t launch_columns3d<double>(const Problem3D *,
                                              const ColumnItem *, int, int,
                                              int, int, int, double *,
-                                             const double *, unsigned int *, hipStream_t);
+                                             const double *, unsigned int *, int, hipStream_t);
 template hipError_t launch_factored_columns3d<float>(const Problem3D *, int,
                                                      int, int, int, int,
                                                      float *, const float *,
@@ -795,7 +815,7 @@ template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
 template hipError_t launch_columns3d<uint32_t>(const Problem3D *,
                                                const ColumnItem *, int, int,
                                                int, int, int, uint32_t *,
-                                               const uint32_t *, unsigned int *, hipStream_t);
+                                               const uint32_t *, unsigned int *, int, hipStream_t);
 template hipError_t launch_factored_columns3d<uint32_t>(const Problem3D *,
                                                         int, int, int, int,
                                                         int, uint32_t *,

@@ -112,6 +112,13 @@ typedef struct
                                       every bulk column-kernel launch of the
                                       first precision segment with HIP
                                       events (ekx_solve_stats.timed_*)      */
+    int chains;                    /* batches: independent launch chains
+                                      (groups of problems on their own
+                                      streams, no barrier between them);
+                                      0 (default) chooses                  */
+    int prefetch_planes;           /* column kernel: z-planes loaded ahead
+                                      of the compute, 4, 8 or 12; 0
+                                      (default) chooses                    */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */

@@ -89,3 +89,39 @@ def test_early_exit_monitor_on_hyperplane_kernel(ekx, layered):
                    convergence_tolerance=0.0, kernel=ekx.Kernel.HYPERPLANE)
     assert np.array_equal(a, b)
     assert sa.last_iteration_change == sb.last_iteration_change
+
+
+@pytest.mark.parametrize("chains", [2, 3, 5])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_chains_give_the_same_fields(ekx, chains, use_graph):
+    """Independent launch chains only reorder work between problems: every
+    problem's FP64 field is bit-identical to the single-chain solve."""
+    rng = np.random.default_rng(21)
+    problems = []
+    shared = None
+    for k in range(7):
+        nx, ny, nz = (int(v) for v in rng.integers(17, 41, 3))
+        g = ekx.Geometry3D(nx, ny, nz, 20.0)
+        if k in (2, 5) and shared is not None:
+            slow = shared
+            g = shared_geom
+        else:
+            slow = 1.0 / rng.uniform(1500.0, 5000.0,
+                                     (nx - 1) * (ny - 1) * (nz - 1))
+            if shared is None:
+                shared, shared_geom = slow, g
+        src = tuple(float(rng.uniform(0, (n - 1) * 20.0))
+                    for n in (g.nx, g.ny, g.nz))
+        problems.append((g, slow, src))
+
+    def run(c):
+        b = ekx.Batch3D()
+        b.initialize(ekx.SolverOptions(number_of_sweeps=2, factored_radius=3,
+                                       chains=c, use_graph=use_graph),
+                     problems)
+        b.solve()
+        return [b.get_travel_time_field(i) for i in range(len(problems))]
+    ref = run(1)
+    got = run(chains)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
