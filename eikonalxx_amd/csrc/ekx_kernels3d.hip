@@ -247,9 +247,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         by = item.by;
     }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
-    // A private copy: the fields the loop uses are loaded once, into
-    // scalar registers, instead of being re-read every step.
-    const Problem3D p = problems[pIndex];
+    const Problem3D &p = problems[pIndex];
     if (!FACTORED)
     {
         const BoxColumns bc = box_columns(p, sweep);
