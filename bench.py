@@ -120,7 +120,9 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                              factored_radius=w["radius"],
                              precision=getattr(ekx.Precision, mode),
                              chains=args.chains,
-                             prefetch_planes=args.prefetch)
+                             prefetch_planes=args.prefetch,
+                             chunk_planes=args.chunk,
+                             use_graph=not args.no_graph)
     batch = ekx.Batch3D()
     batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
     for i in range(args.warmup):
@@ -172,7 +174,8 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                                   factored_radius=w["radius"],
                                   precision=getattr(ekx.Precision, mode),
                                   kernel_timing=True, chains=args.chains,
-                                  prefetch_planes=args.prefetch)
+                                  prefetch_planes=args.prefetch,
+                                  chunk_planes=args.chunk)
         tb = ekx.Batch3D()
         tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
         tb.solve()
@@ -260,6 +263,12 @@ def main():
     ap.add_argument("--prefetch", type=int, default=0,
                     help="column-kernel planes prefetched ahead (4, 8, 12; "
                          "0: the library's choice)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="issue the sweep launches eagerly instead of "
+                         "replaying a captured hipGraph")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="z-planes per column chunk (0: the library's "
+                         "choice)")
     ap.add_argument("--chains", type=int, default=0,
                     help="independent launch chains (0: the library's "
                          "choice)")

@@ -78,7 +78,8 @@ class _Options(ctypes.Structure):
                 ("use_graph", ctypes.c_int),
                 ("kernel_timing", ctypes.c_int),
                 ("chains", ctypes.c_int),
-                ("prefetch_planes", ctypes.c_int)]
+                ("prefetch_planes", ctypes.c_int),
+                ("chunk_planes", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -256,7 +257,7 @@ class SolverOptions:
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
                  use_graph=True, verbosity=0, kernel_timing=False,
-                 chains=0, prefetch_planes=0):
+                 chains=0, prefetch_planes=0, chunk_planes=0):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -275,6 +276,7 @@ class SolverOptions:
         o.kernel_timing = 1 if kernel_timing else 0
         o.chains = int(chains)
         o.prefetch_planes = int(prefetch_planes)
+        o.chunk_planes = int(chunk_planes)
         o.verbosity = int(verbosity)
         self._o = o
 

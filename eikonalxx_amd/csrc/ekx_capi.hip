@@ -157,6 +157,10 @@ ekx_status check_options(const ekx_solver_options *o)
         return fail(EKX_EINVAL, "chains must be in [0, " +
                                     std::to_string(kMaxChains) + "]");
     }
+    if (o->chunk_planes < 0)
+    {
+        return fail(EKX_EINVAL, "chunk_planes must be >= 0");
+    }
     if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
         o->prefetch_planes != 8 && o->prefetch_planes != 12)
     {
@@ -268,30 +272,39 @@ HostOctant host_octant(const Problem3D &p, int sweep, bool initialize)
     return o;
 }
 
-// Updates a sweep makes in the columns that overlap the factored box (the
-// work of the factored column kernel; box_columns in ekx_kernels3d.hip).
-int64_t host_box_updates(const Problem3D &p, int sweep, bool initialize)
+// Updates a sweep makes in the column chunks that overlap the factored box
+// (the work of the factored column kernel; box_columns in
+// ekx_kernels3d.hip).
+int64_t host_box_updates(const Problem3D &p, int sweep, bool initialize,
+                         int chunk)
 {
     if (p.radius < 0) { return 0; }
     const int negX = sweep & 1, negY = (sweep >> 1) & 1;
+    const int negZ = (sweep >> 2) & 1;
     int xLo = std::max(p.srcCellX - p.radius, 0);
     int xHi = std::min(p.srcCellX + p.radius, p.nx - 1);
     int yLo = std::max(p.srcCellY - p.radius, 0);
     int yHi = std::min(p.srcCellY + p.radius, p.ny - 1);
-    if (xLo > xHi || yLo > yHi) { return 0; }
+    int zLo = std::max(p.srcCellZ - p.radius, 0);
+    int zHi = std::min(p.srcCellZ + p.radius, p.nz - 1);
+    if (xLo > xHi || yLo > yHi || zLo > zHi) { return 0; }
     int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
     int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
+    int jzLo = negZ ? p.nz - 1 - zHi : zLo, jzHi = negZ ? p.nz - 1 - zLo : zHi;
     const HostOctant o = host_octant(p, sweep, initialize);
-    // nodes of columns bLo..bHi (8 wide) at or past j0, below n
-    auto span = [](int bLo, int bHi, int j0, int n)
+    // nodes of blocks bLo..bHi (w wide) at or past j0, below n
+    auto span = [](int bLo, int bHi, int w, int j0, int n)
     {
-        int lo = std::max(bLo*kColumnTile, j0);
-        int hi = std::min(bHi*kColumnTile + kColumnTile - 1, n - 1);
+        int lo = std::max(bLo*w, j0);
+        int hi = std::min(bHi*w + w - 1, n - 1);
         return std::max(0, hi - lo + 1);
     };
-    int64_t nxs = span(jxLo/kColumnTile, jxHi/kColumnTile, o.jx0, p.nx);
-    int64_t nys = span(jyLo/kColumnTile, jyHi/kColumnTile, o.jy0, p.ny);
-    return nxs*nys*(int64_t) (p.nz - o.jz0);
+    int64_t nxs = span(jxLo/kColumnTile, jxHi/kColumnTile, kColumnTile,
+                       o.jx0, p.nx);
+    int64_t nys = span(jyLo/kColumnTile, jyHi/kColumnTile, kColumnTile,
+                       o.jy0, p.ny);
+    int64_t nzs = span(jzLo/chunk, jzHi/chunk, chunk, o.jz0, p.nz);
+    return nxs*nys*nzs;
 }
 
 } // namespace
@@ -331,6 +344,7 @@ struct Engine3D
     // a contiguous range of problems [chainBegin[c], chainBegin[c+1]).
     std::vector<int> diagStart;
     int nDiag = 0;
+    int chunkH = 1;      // planes per z-chunk (>= nz: whole columns)
     int nChains = 1;
     std::vector<int> chainBegin;
     std::vector<hipStream_t> chainStreams;
@@ -338,6 +352,9 @@ struct Engine3D
     // Upper bound, over any source position, of the columns per diagonal
     // that overlap one problem's factored box: the box spans at most
     // ceil((2r+1)/8)+1 columns per axis.
+    // Upper bound of the chunks per problem that overlap the factored box
+    // on one diagonal: the box's columns in x times those in y (each
+    // (bx, by) meets a diagonal at one bz at most).
     int boxPerDiag = 0;
     hipStream_t stream = nullptr;
     hipStream_t sideStream = nullptr;   // factored-box columns
@@ -574,14 +591,18 @@ struct Engine3D
             chainStreams.push_back(cs);
             chainEvents.push_back(ce);
         }
-        // Column work lists by chain, then diagonal bx + by.
+        // Column-chunk work lists by chain, then diagonal bx + by + bz.
+        int maxNz = 0;
+        for (int i = 0; i < n; ++i) { maxNz = std::max(maxNz, geoms[i].nz); }
+        chunkH = o.chunk_planes > 0 ? std::min(o.chunk_planes, maxNz) : maxNz;
         nDiag = 0;
         for (int i = 0; i < n; ++i)
         {
             const ekx_geometry3d &g = geoms[i];
             int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
             int nby = (g.ny + kColumnTile - 1)/kColumnTile;
-            nDiag = std::max(nDiag, nbx + nby - 1);
+            int nbz = (g.nz + chunkH - 1)/chunkH;
+            nDiag = std::max(nDiag, nbx + nby + nbz - 2);
         }
         std::vector<ColumnItem> flat;
         diagStart.clear();
@@ -593,11 +614,16 @@ struct Engine3D
                 const ekx_geometry3d &g = geoms[i];
                 int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
                 int nby = (g.ny + kColumnTile - 1)/kColumnTile;
-                for (int by = 0; by < nby; ++by)
+                int nbz = (g.nz + chunkH - 1)/chunkH;
+                for (int bz = 0; bz < nbz; ++bz)
                 {
-                    for (int bx = 0; bx < nbx; ++bx)
+                    for (int by = 0; by < nby; ++by)
                     {
-                        perDiag[bx + by].push_back(ColumnItem{i, bx, by, 0});
+                        for (int bx = 0; bx < nbx; ++bx)
+                        {
+                            perDiag[bx + by + bz].push_back(
+                                ColumnItem{i, bx, by, bz});
+                        }
                     }
                 }
             }
@@ -611,6 +637,8 @@ struct Engine3D
         boxPerDiag = 0;
         if (o.factored_radius >= 0)
         {
+            // a box 2r+1 nodes wide touches at most ceil((2r+1)/8)+1
+            // columns per axis
             int span = (2*o.factored_radius + 1 + kColumnTile - 1)/kColumnTile
                      + 1;
             for (int i = 0; i < n; ++i)
@@ -619,7 +647,7 @@ struct Engine3D
                 int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
                 int nby = (g.ny + kColumnTile - 1)/kColumnTile;
                 boxPerDiag = std::max(boxPerDiag,
-                                      std::min(span, std::min(nbx, nby)));
+                                      std::min(span, nbx)*std::min(span, nby));
             }
         }
         EKX_CHECK(dItems.allocate(flat.size()));
@@ -755,7 +783,7 @@ struct Engine3D
             if (kernel() == EKX_KERNEL_COLUMN)
             {
                 st.updates_factored += host_box_updates(p, sweep,
-                                                        initialize);
+                                                        initialize, chunkH);
             }
         }
     }
@@ -792,8 +820,8 @@ struct Engine3D
                 if (timing) { EKX_CHECK(record_timing_event(cs)); }
                 EKX_HIP(launch_columns3d<TS>(dProblems.ptr, dItems.ptr + ds[d],
                                              count, sweep, initialize ? 1 : 0,
-                                             d, boxPerDiag, T, S, change.ptr,
-                                             depth(), cs));
+                                             d, boxPerDiag, chunkH, T, S,
+                                             change.ptr, depth(), cs));
                 if (timing) { EKX_CHECK(record_timing_event(cs)); }
                 launches += 1;
             }
@@ -801,7 +829,7 @@ struct Engine3D
             {
                 EKX_HIP(launch_factored_columns3d<TS>(
                     dProblems.ptr + p0, np, sweep, initialize ? 1 : 0, d,
-                    boxPerDiag, T, S, change.ptr, cs));
+                    boxPerDiag, chunkH, T, S, change.ptr, cs));
                 launches += 1;
             }
         }
@@ -847,15 +875,15 @@ struct Engine3D
                     EKX_HIP(hipStreamWaitEvent(sideStream, evFork, 0));
                     EKX_HIP(launch_factored_columns3d<TS>(
                         dProblems.ptr, n, sweep, initialize ? 1 : 0, d,
-                        boxPerDiag, T, S, change.ptr, sideStream));
+                        boxPerDiag, chunkH, T, S, change.ptr, sideStream));
                     launches += 1;
                 }
                 if (timing) { EKX_CHECK(record_timing_event(stream)); }
                 EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
                                              dItems.ptr + diagStart[d], count,
                                              sweep, initialize ? 1 : 0, d,
-                                             boxPerDiag, T, S, change.ptr,
-                                             depth(), stream));
+                                             boxPerDiag, chunkH, T, S,
+                                             change.ptr, depth(), stream));
                 if (timing) { EKX_CHECK(record_timing_event(stream)); }
                 launches += 1;
                 if (boxPerDiag > 0)
@@ -1213,6 +1241,7 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->kernel_timing = 0;
     o->chains = 0;
     o->prefetch_planes = 0;
+    o->chunk_planes = 0;
 }
 
 ekx_status ekx_device_count(int *count)

@@ -50,12 +50,11 @@ struct Problem3D
     NonUniform3D nu;
 };
 
-// A column (8 x 8 footprint, all z) of one problem.
+// A column chunk (8 x 8 footprint, one z-chunk) of one problem.
 struct ColumnItem
 {
     int problem;
-    int bx, by;                 // column index in the permuted frame
-    int pad;
+    int bx, by, bz;             // column and chunk index, permuted frame
 };
 
 // One problem of a 2D batch.

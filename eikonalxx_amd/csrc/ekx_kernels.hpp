@@ -22,23 +22,27 @@ hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,
                                int nPairs, TS *T, const TS *S,
                                unsigned int *change, hipStream_t stream);
-// One column diagonal, Cartesian operators: the diagonal's items, except
-// the columns overlapping a factored box (boxPerDiag > 0).  depth: planes
-// prefetched ahead of the compute (4, 8 or 12).
+// One diagonal bx + by + bz of column chunks (8 x 8 columns cut into
+// z-chunks of `chunk` planes), Cartesian operators: the diagonal's items,
+// except the chunks overlapping a factored box (boxPerDiag > 0).  depth:
+// planes prefetched ahead of the compute (4, 8 or 12).
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
-                            TS *T, const TS *S, unsigned int *change,
-                            int depth, hipStream_t stream);
-// The same diagonal's columns that overlap a factored box: up to
-// boxPerDiag per problem, full local update.  Independent of the launch
-// above (disjoint columns), so the two may run concurrently.
+                            int chunk, TS *T, const TS *S,
+                            unsigned int *change, int depth,
+                            hipStream_t stream);
+// The same diagonal's chunks that overlap a factored box: up to
+// boxPerDiag per problem (the box's columns in x and y), full local
+// update.  Independent of the launch above (disjoint chunks), so the two
+// may run concurrently.
 template <typename TS>
 hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      int nProblems, int sweep, int initialize,
-                                     int diagonal, int boxPerDiag, TS *T,
-                                     const TS *S, unsigned int *change,
+                                     int diagonal, int boxPerDiag, int chunk,
+                                     TS *T, const TS *S,
+                                     unsigned int *change,
                                      hipStream_t stream);
 // Sweep launches take `change`: NULL, or a device word that receives (as
 // float bits, by atomic max) the largest decrease any update of the launch

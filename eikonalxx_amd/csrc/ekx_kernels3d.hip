@@ -37,30 +37,37 @@ __device__ __forceinline__ Octant sweep_octant(const Problem3D &p, int sweep,
 }
 
 // Columns (8 x 8 footprints in the permuted frame) that overlap the
-// factored-eikonal box |i - iSrc| <= radius (solverUtilities3d.hpp:296-298).
-// Empty (lo > hi) when the radius is negative.
+// factored-eikonal box |i - iSrc| <= radius (solverUtilities3d.hpp:296-298),
+// in units of columns (x, y) and z-chunks of `chunk` planes.  Empty
+// (lo > hi) when the radius is negative.
 struct BoxColumns
 {
-    int bxLo, bxHi, byLo, byHi;
+    int bxLo, bxHi, byLo, byHi, bzLo, bzHi;
 };
 
 __device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
-                                                  int sweep)
+                                                  int sweep, int chunk)
 {
-    BoxColumns c{1, 0, 1, 0};
+    BoxColumns c{1, 0, 1, 0, 1, 0};
     if (p.radius < 0) { return c; }
     const int negX = sweep & 1, negY = (sweep >> 1) & 1;
+    const int negZ = (sweep >> 2) & 1;
     int xLo = max(p.srcCellX - p.radius, 0);
     int xHi = min(p.srcCellX + p.radius, p.nx - 1);
     int yLo = max(p.srcCellY - p.radius, 0);
     int yHi = min(p.srcCellY + p.radius, p.ny - 1);
-    if (xLo > xHi || yLo > yHi) { return c; }
+    int zLo = max(p.srcCellZ - p.radius, 0);
+    int zHi = min(p.srcCellZ + p.radius, p.nz - 1);
+    if (xLo > xHi || yLo > yHi || zLo > zHi) { return c; }
     int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
     int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
+    int jzLo = negZ ? p.nz - 1 - zHi : zLo, jzHi = negZ ? p.nz - 1 - zLo : zHi;
     c.bxLo = jxLo/kColumnTile;
     c.bxHi = jxHi/kColumnTile;
     c.byLo = jyLo/kColumnTile;
     c.byHi = jyHi/kColumnTile;
+    c.bzLo = jzLo/chunk;
+    c.bzHi = jzHi/chunk;
     return c;
 }
 
@@ -214,7 +221,7 @@ template <typename TS, int WAVES, int DL, bool FACTORED>
 __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
                 const ColumnItem *__restrict__ items, int nItems, int sweep,
-                int initialize, int diagonal, int boxPerDiag,
+                int initialize, int diagonal, int boxPerDiag, int chunk,
                 TS *__restrict__ T, const TS *__restrict__ S,
                 unsigned int *__restrict__ change)
 {
@@ -226,18 +233,20 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int lane = threadIdx.x & 63;
     const int itemIndex = blockIdx.x*WAVES + wave;
     if (itemIndex >= nItems) { return; }
-    int pIndex, bx, by;
+    int pIndex, bx, by, bz;
     if (FACTORED)
     {
-        // item = (problem, k): the k-th box column on this diagonal
+        // item = (problem, k): the k-th (bx, by) of the box, at the z-chunk
+        // that puts it on this diagonal bx + by + bz
         pIndex = itemIndex/boxPerDiag;
         const int k = itemIndex - pIndex*boxPerDiag;
-        const BoxColumns bc = box_columns(problems[pIndex], sweep);
-        const int lo = max(bc.bxLo, diagonal - bc.byHi);
-        const int hi = min(bc.bxHi, diagonal - bc.byLo);
-        if (lo + k > hi) { return; }
-        bx = lo + k;
-        by = diagonal - bx;
+        const BoxColumns bc = box_columns(problems[pIndex], sweep, chunk);
+        const int nbx = bc.bxHi - bc.bxLo + 1, nby = bc.byHi - bc.byLo + 1;
+        if (nbx <= 0 || nby <= 0 || k >= nbx*nby) { return; }
+        bx = bc.bxLo + k%nbx;
+        by = bc.byLo + k/nbx;
+        bz = diagonal - bx - by;
+        if (bz < bc.bzLo || bz > bc.bzHi) { return; }
     }
     else
     {
@@ -245,14 +254,15 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         pIndex = item.problem;
         bx = item.bx;
         by = item.by;
+        bz = item.bz;
     }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
     const Problem3D &p = problems[pIndex];
     if (!FACTORED)
     {
-        const BoxColumns bc = box_columns(p, sweep);
+        const BoxColumns bc = box_columns(p, sweep, chunk);
         if (bx >= bc.bxLo && bx <= bc.bxHi && by >= bc.byLo &&
-            by <= bc.byHi)
+            by <= bc.byHi && bz >= bc.bzLo && bz <= bc.bzHi)
         {
             return;   // handled by the FACTORED instance
         }
@@ -267,8 +277,14 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     {
         return;   // column entirely outside the source octant
     }
-    const int P0 = o.jz0 - 1;             // first plane: read only
-    const int nPlanes = nz - P0;
+    // This item's planes: the chunk's [bz*chunk, (bz+1)*chunk) within the
+    // sweep's range, after a read-only first plane (the previous chunk's
+    // last, or the plane before the range).
+    const int zFirst = max(__builtin_amdgcn_readfirstlane(bz)*chunk, o.jz0);
+    const int zEnd = min((__builtin_amdgcn_readfirstlane(bz) + 1)*chunk, nz);
+    if (zFirst >= zEnd) { return; }
+    const int P0 = zFirst - 1;            // first plane: read only
+    const int nPlanes = zEnd - P0;
     const int a = lane & 7, b = lane >> 3;
     const int jx = X0 + a, jy = Y0 + b;
     const bool valid = jx < nx && jy < ny;
@@ -665,8 +681,9 @@ template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
                             int initialize, int diagonal, int boxPerDiag,
-                            TS *T, const TS *S, unsigned int *change,
-                            int depth, hipStream_t stream)
+                            int chunk, TS *T, const TS *S,
+                            unsigned int *change, int depth,
+                            hipStream_t stream)
 {
     if (nItems <= 0) { return hipSuccess; }
     constexpr int W = column_waves<TS>();
@@ -676,17 +693,20 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
     case 4:
         hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>), grid, block,
                            0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, T, S, change);
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
+                           change);
         break;
     case 8:
         hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false>), grid, block,
                            0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, T, S, change);
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
+                           change);
         break;
     case 12:
         hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false>), grid, block,
                            0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, T, S, change);
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
+                           change);
         break;
     default:
         return hipErrorInvalidValue;
@@ -697,8 +717,9 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
 template <typename TS>
 hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      int nProblems, int sweep, int initialize,
-                                     int diagonal, int boxPerDiag, TS *T,
-                                     const TS *S, unsigned int *change,
+                                     int diagonal, int boxPerDiag, int chunk,
+                                     TS *T, const TS *S,
+                                     unsigned int *change,
                                      hipStream_t stream)
 {
     if (boxPerDiag <= 0) { return hipSuccess; }
@@ -706,7 +727,7 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
     hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true>), dim3((unsigned) n),
                        dim3(64), 0, stream, dProblems,
                        (const ColumnItem *) nullptr, n, sweep, initialize,
-                       diagonal, boxPerDiag, T, S, change);
+                       diagonal, boxPerDiag, chunk, T, S, change);
     return hipGetLastError();
 }
 
@@ -786,23 +807,6 @@ template hipError_t launch_hyperplane3d<float>(const Problem3D *, int, int,
 template hipError_t launch_hyperplane3d<double>(const Problem3D *, int, int,
                                                 int, int, int, int, double *,
                                                 const double *, unsigned int *, hipStream_t);
-template hipError_t launch_columns3d<float>(const Problem3D *,
-                                            const ColumnItem *, int, int, int,
-                                            int, int, float *, const float *,
-                                            unsigned int *, int, hipStream_t);
-template hipError_t launch_columns3d<double>(const Problem3D *,
-                                             const ColumnItem *, int, int,
-                                             int, int, int, double *,
-                                             const double *, unsigned int *, int, hipStream_t);
-template hipError_t launch_factored_columns3d<float>(const Problem3D *, int,
-                                                     int, int, int, int,
-                                                     float *, const float *,
-                                                     unsigned int *, hipStream_t);
-template hipError_t launch_factored_columns3d<double>(const Problem3D *, int,
-                                                      int, int, int, int,
-                                                      double *,
-                                                      const double *,
-                                                      unsigned int *, hipStream_t);
 template hipError_t launch_init3d<uint32_t>(const Problem3D *, int, int64_t,
                                             uint32_t *, hipStream_t);
 template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
@@ -810,14 +814,24 @@ template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
                                                   uint32_t *,
                                                   const uint32_t *,
                                                   unsigned int *, hipStream_t);
-template hipError_t launch_columns3d<uint32_t>(const Problem3D *,
-                                               const ColumnItem *, int, int,
-                                               int, int, int, uint32_t *,
-                                               const uint32_t *, unsigned int *, int, hipStream_t);
-template hipError_t launch_factored_columns3d<uint32_t>(const Problem3D *,
-                                                        int, int, int, int,
-                                                        int, uint32_t *,
-                                                        const uint32_t *,
-                                                        unsigned int *, hipStream_t);
+
+template hipError_t launch_columns3d<float>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    float *, const float *, unsigned int *, int, hipStream_t);
+template hipError_t launch_factored_columns3d<float>(
+    const Problem3D *, int, int, int, int, int, int, float *, const float *,
+    unsigned int *, hipStream_t);
+template hipError_t launch_columns3d<double>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    double *, const double *, unsigned int *, int, hipStream_t);
+template hipError_t launch_factored_columns3d<double>(
+    const Problem3D *, int, int, int, int, int, int, double *, const double *,
+    unsigned int *, hipStream_t);
+template hipError_t launch_columns3d<uint32_t>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    uint32_t *, const uint32_t *, unsigned int *, int, hipStream_t);
+template hipError_t launch_factored_columns3d<uint32_t>(
+    const Problem3D *, int, int, int, int, int, int, uint32_t *, const uint32_t *,
+    unsigned int *, hipStream_t);
 
 } // namespace ekx

@@ -119,6 +119,11 @@ typedef struct
     int prefetch_planes;           /* column kernel: z-planes loaded ahead
                                       of the compute, 4, 8 or 12; 0
                                       (default) chooses                    */
+    int chunk_planes;              /* column kernel: z-planes per chunk of
+                                      a column (chunks run on diagonals
+                                      bx+by+bz, shortening the dependency
+                                      chain of a sweep); 0 (default):
+                                      whole columns                        */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */

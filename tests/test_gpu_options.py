@@ -125,3 +125,45 @@ def test_chains_give_the_same_fields(ekx, chains, use_graph):
     got = run(chains)
     for a, b in zip(ref, got):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("chunk", [1, 5, 16, 33])
+def test_chunked_columns_give_the_same_field(ekx, layered, chunk):
+    """Cutting columns into z-chunks (diagonals bx + by + bz) keeps every
+    node's upwind neighbours ahead of it: the FP64 field is unchanged."""
+    w, slow = layered
+    a, sa = _solve(ekx, w, slow, number_of_sweeps=2)
+    b, sb = _solve(ekx, w, slow, number_of_sweeps=2, chunk_planes=chunk)
+    assert np.array_equal(a, b)
+    assert sa.updates == sb.updates
+    c, _ = _solve(ekx, w, slow, number_of_sweeps=1, chunk_planes=chunk,
+                  schedule=ekx.Schedule.SHIPPED,
+                  algorithm=ekx.SolverAlgorithm.FastSweepingMethod)
+    d, _ = _solve(ekx, w, slow, number_of_sweeps=1,
+                  schedule=ekx.Schedule.SHIPPED,
+                  algorithm=ekx.SolverAlgorithm.FastSweepingMethod)
+    assert np.array_equal(c, d)
+
+
+@pytest.mark.parametrize("chunk,chains", [(7, 1), (12, 3)])
+def test_chunked_batch(ekx, oracle, chunk, chains):
+    """Uneven batch, chunked and chained, against the oracle bit for bit."""
+    rng = np.random.default_rng(33)
+    problems = []
+    for k in range(4):
+        nx, ny, nz = (int(v) for v in rng.integers(19, 37, 3))
+        g = ekx.Geometry3D(nx, ny, nz, 15.0)
+        slow = 1.0 / rng.uniform(1500.0, 5000.0,
+                                 (nx - 1) * (ny - 1) * (nz - 1))
+        src = tuple(float(rng.uniform(0, (n - 1) * 15.0))
+                    for n in (nx, ny, nz))
+        problems.append((g, slow, src))
+    b = ekx.Batch3D()
+    b.initialize(ekx.SolverOptions(number_of_sweeps=2, factored_radius=3,
+                                   chunk_planes=chunk, chains=chains),
+                 problems)
+    b.solve()
+    for i, (g, slow, src) in enumerate(problems):
+        og = oracle.geometry3d(g.nx, g.ny, g.nz, 15.0)
+        ref = oracle.solve3d(og, slow, src, 2, 3)
+        assert np.array_equal(b.get_travel_time_field(i), ref)
