@@ -76,6 +76,12 @@ def _declare(L):
                                              _D, _D, _PD, _PD, _PD, _PD]
     L.ekxo_analytic_time2d.restype = _D
     L.ekxo_analytic_time2d.argtypes = [_I, _I, _D, _D, _D, _D, _D]
+    L.ekxo_station_time3d.restype = _D
+    L.ekxo_station_time3d.argtypes = [ctypes.POINTER(Geometry3D), _PD, _PD,
+                                      _D, _D, _D, _D, _D, _D]
+    for name, geo in (("ekxo_gradient2d", Geometry2D),
+                      ("ekxo_gradient3d", Geometry3D)):
+        getattr(L, name).restype = _I
     L.ekxo_station_time2d.restype = _D
     L.ekxo_station_time2d.argtypes = [ctypes.POINTER(Geometry2D), _PD, _PD,
                                       _D, _D, _D, _D]
@@ -449,3 +455,35 @@ def fd2d_nonuniform(radius, dxz, s_src, ik, signs, shifts, isrc, src_off,
     t = (ctypes.c_double * 3)(*t3)
     return lib().ekxo_fd2d_nonuniform(radius, huge, dxz[0], dxz[1], s_src,
                                       ik[0], ik[1], sg, sh, isr, off, s, t)
+
+
+def station_time3d(g, slowness, T, source, receiver):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    return lib().ekxo_station_time3d(ctypes.byref(g), _dp(s), _dp(T),
+                                     *[_D(v) for v in source],
+                                     *[_D(v) for v in receiver])
+
+
+def gradient2d(g, slowness, T, source):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    out = np.empty(2 * g.nx * g.nz)
+    rc = lib().ekxo_gradient2d(ctypes.byref(g), _dp(s), _dp(T),
+                               _D(source[0]), _D(source[1]), _dp(out))
+    if rc != 0:
+        raise ValueError("source outside the model" if rc == -1
+                         else "at least 3 grid points per axis")
+    return out
+
+
+def gradient3d(g, slowness, T, source):
+    s = np.ascontiguousarray(slowness, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    out = np.empty(3 * g.nx * g.ny * g.nz)
+    rc = lib().ekxo_gradient3d(ctypes.byref(g), _dp(s), _dp(T),
+                               *[_D(v) for v in source], _dp(out))
+    if rc != 0:
+        raise ValueError("source outside the model" if rc == -1
+                         else "at least 3 grid points per axis")
+    return out

@@ -1645,3 +1645,203 @@ double ekxo_station_time2d(const ekxo_geometry2d *g, const double *slow,
     double zmz1 = zi - z0;
     return dzi*(z2mz*fxz1 + zmz1*fxz2);
 }
+
+/* ======================================================================== */
+/*                    Stations and travel-time gradients                    */
+/* ======================================================================== */
+
+/* 3D travel time at a receiver: the 2D rule of src/solver2d.cpp:106-167
+ * (with src/ray/bilinear.hpp) extended to three axes -- the analytic time
+ * with the source-cell slowness when the receiver shares the source cell,
+ * else trilinear interpolation of the cell's 8 corner times, interpolating
+ * in x, then y, then z. */
+double ekxo_station_time3d(const ekxo_geometry3d *g, const double *slow,
+                           const double *T, double xs, double ys, double zs,
+                           double xr, double yr, double zr)
+{
+    double soff[3], roff[3];
+    int scell[3], rcell[3], sIndex, rIndex;
+    if (ekxo_source3d(g, xs, ys, zs, soff, scell, &sIndex) != 0) { return -1; }
+    if (ekxo_source3d(g, xr, yr, zr, roff, rcell, &rIndex) != 0) { return -1; }
+    double xi = roff[0], yi = roff[1], zi = roff[2];
+    if (rIndex == sIndex)
+    {
+        double ddx = xi - soff[0], ddy = yi - soff[1], ddz = zi - soff[2];
+        return sqrt(ddx*ddx + ddy*ddy + ddz*ddz)*slow[sIndex];
+    }
+    double dx = g->dx, dy = g->dy, dz = g->dz;
+    double dxi = 1./dx, dyi = 1./dy, dzi = 1./dz;
+    size_t i000 = grid3(g->nx, g->ny, rcell[0], rcell[1], rcell[2]);
+    size_t sy = (size_t) g->nx, sz = (size_t) g->nx*g->ny;
+    double x0 = rcell[0]*dx, x1 = (rcell[0] + 1)*dx;
+    double y0 = rcell[1]*dy, y1 = (rcell[1] + 1)*dy;
+    double z0 = rcell[2]*dz, z1 = (rcell[2] + 1)*dz;
+    double x2mx = x1 - xi, xmx1 = xi - x0;
+    double y2my = y1 - yi, ymy1 = yi - y0;
+    double z2mz = z1 - zi, zmz1 = zi - z0;
+    double f[2];
+    for (int k = 0; k < 2; ++k)
+    {
+        size_t b = i000 + k*sz;
+        double fy0 = dxi*(x2mx*T[b] + xmx1*T[b + 1]);
+        double fy1 = dxi*(x2mx*T[b + sy] + xmx1*T[b + sy + 1]);
+        f[k] = dyi*(y2my*fy0 + ymy1*fy1);
+    }
+    return dzi*(z2mz*f[0] + zmz1*f[1]);
+}
+
+/* src/gradient.hpp:19-260 (CentralDifference): central differences inside,
+ * second-order one-sided differences on the edges, the analytic gradient
+ * (source-cell slowness) at the 4 corners of the source cell when the
+ * source is off its lower corner.  grad holds (dT/dx, dT/dz) per node.
+ * Returns -1 for a source outside the model, -2 for fewer than 3 grid
+ * points on an axis. */
+int ekxo_gradient2d(const ekxo_geometry2d *g, const double *slow,
+                    const double *T, double xsAbs, double zsAbs,
+                    double *grad)
+{
+    double off[2];
+    int cell[2], cellIndex;
+    if (ekxo_source2d(g, xsAbs, zsAbs, off, cell, &cellIndex) != 0)
+    {
+        return -1;
+    }
+    const int nx = g->nx, nz = g->nz;
+    if (nx < 3 || nz < 3) { return -2; }   /* src/gradient.hpp:31-41 */
+    const double dx = g->dx, dz = g->dz;
+    const double twodxi = 1./(2.*dx), twodzi = 1./(2.*dz);
+    for (int iz = 0; iz < nz; ++iz)
+    {
+        for (int ix = 0; ix < nx; ++ix)
+        {
+            size_t i = (size_t) grid2(nx, ix, iz);
+            double u = T[i], gx, gz;
+            if (ix == 0)
+            {
+                gx = (-3*u + 4*T[i + 1] - T[i + 2])*twodxi;
+            }
+            else if (ix == nx - 1)
+            {
+                gx = (T[i - 2] - 4*T[i - 1] + 3*u)*twodxi;
+            }
+            else
+            {
+                gx = (T[i + 1] - T[i - 1])*twodxi;
+            }
+            if (iz == 0)
+            {
+                gz = (-3*u + 4*T[i + nx] - T[i + 2*nx])*twodzi;
+            }
+            else if (iz == nz - 1)
+            {
+                gz = (T[i - 2*nx] - 4*T[i - nx] + 3*u)*twodzi;
+            }
+            else
+            {
+                gz = (T[i + nx] - T[i - nx])*twodzi;
+            }
+            grad[2*i] = gx;
+            grad[2*i + 1] = gz;
+        }
+    }
+    const double xs = off[0], zs = off[1];
+    const double slow0 = slow[cellIndex];
+    const double tol = 1.e-7;
+    double dxs = (double) cell[0]*dx - xs;
+    double dzs = (double) cell[1]*dz - zs;
+    if (fabs(dxs) > tol || fabs(dzs) > tol)
+    {
+        int doX = fabs(dxs) > tol, doZ = fabs(dzs) > tol;
+        for (int k = 0; k < 2; ++k)
+        {
+            for (int j = 0; j < 2; ++j)
+            {
+                double ddx = (cell[0] + j)*dx - xs;
+                double ddz = (cell[1] + k)*dz - zs;
+                double r = sqrt(ddx*ddx + ddz*ddz);
+                size_t i = (size_t) grid2(nx, cell[0] + j, cell[1] + k);
+                if (doX) { grad[2*i] = (ddx/r)*slow0; }
+                if (doZ) { grad[2*i + 1] = (ddz/r)*slow0; }
+            }
+        }
+    }
+    return 0;
+}
+
+/* The same rule on three axes; grad holds (dT/dx, dT/dy, dT/dz) per node.
+ * The reference computes no 3D gradient (solver3d.hpp:153-163 only declares
+ * the accessors); this is the 2D rule of src/gradient.hpp per axis. */
+int ekxo_gradient3d(const ekxo_geometry3d *g, const double *slow,
+                    const double *T, double xsAbs, double ysAbs,
+                    double zsAbs, double *grad)
+{
+    double off[3];
+    int cell[3], cellIndex;
+    if (ekxo_source3d(g, xsAbs, ysAbs, zsAbs, off, cell, &cellIndex) != 0)
+    {
+        return -1;
+    }
+    if (g->nx < 3 || g->ny < 3 || g->nz < 3) { return -2; }
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const double d[3] = {g->dx, g->dy, g->dz};
+    const size_t stride[3] = {1, (size_t) g->nx, (size_t) g->nx*g->ny};
+    for (int iz = 0; iz < g->nz; ++iz)
+    {
+        for (int iy = 0; iy < g->ny; ++iy)
+        {
+            for (int ix = 0; ix < g->nx; ++ix)
+            {
+                size_t i = grid3(g->nx, g->ny, ix, iy, iz);
+                const int idx[3] = {ix, iy, iz};
+                for (int a = 0; a < 3; ++a)
+                {
+                    const double two_di = 1./(2.*d[a]);
+                    const size_t s = stride[a];
+                    double u = T[i], v;
+                    if (idx[a] == 0)
+                    {
+                        v = (-3*u + 4*T[i + s] - T[i + 2*s])*two_di;
+                    }
+                    else if (idx[a] == n[a] - 1)
+                    {
+                        v = (T[i - 2*s] - 4*T[i - s] + 3*u)*two_di;
+                    }
+                    else
+                    {
+                        v = (T[i + s] - T[i - s])*two_di;
+                    }
+                    grad[3*i + a] = v;
+                }
+            }
+        }
+    }
+    const double slow0 = slow[cellIndex];
+    const double tol = 1.e-7;
+    int doAxis[3], any = 0;
+    for (int a = 0; a < 3; ++a)
+    {
+        double ds = (double) cell[a]*d[a] - off[a];
+        doAxis[a] = fabs(ds) > tol;
+        any |= doAxis[a];
+    }
+    if (any)
+    {
+        for (int c = 0; c < 8; ++c)
+        {
+            int o[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+            double dd[3];
+            for (int a = 0; a < 3; ++a)
+            {
+                dd[a] = (cell[a] + o[a])*d[a] - off[a];
+            }
+            double r = sqrt(dd[0]*dd[0] + dd[1]*dd[1] + dd[2]*dd[2]);
+            size_t i = grid3(g->nx, g->ny, cell[0] + o[0], cell[1] + o[1],
+                             cell[2] + o[2]);
+            for (int a = 0; a < 3; ++a)
+            {
+                if (doAxis[a]) { grad[3*i + a] = (dd[a]/r)*slow0; }
+            }
+        }
+    }
+    return 0;
+}

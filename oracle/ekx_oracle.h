@@ -27,6 +27,13 @@
  *       return T0 on exact T0 data), so only first-order convergence of
  *       whole solves to the analytic fields and the line-by-line
  *       restatement pin them.
+ *   2D station interpolation   -> testing/solver2d.cpp:1195-1302 values.
+ *   3D station interpolation   -> exact on multilinear fields, analytic in
+ *       the source cell, equal to the 2D rule on y-invariant fields.
+ *   2D/3D gradients            -> testing/gradient.cpp (homogeneous 5e-5,
+ *       linear gradient 1e-4 against the analytic gradients); exact on
+ *       quadratics (every edge stencil); 3D equals 2D on y-invariant
+ *       fields; the source-corner values against the closed form.
  *   Bitwise parity with the reference binary itself is unpinned: the
  *   reference needs SYCL and TBB headers that this image lacks.
  */
@@ -186,6 +193,17 @@ int ekxo_homogeneous2d(const ekxo_geometry2d *g, double xs, double zs,
                        double vel, double *T);
 int ekxo_linear_gradient2d(const ekxo_geometry2d *g, double xs, double zs,
                            double vTop, double vBottom, double *T);
+/* Trilinear travel time at a receiver (the 2D rule extended). */
+double ekxo_station_time3d(const ekxo_geometry3d *g, const double *slow,
+                           const double *T, double xs, double ys, double zs,
+                           double xr, double yr, double zr);
+/* Travel-time gradients (src/gradient.hpp, central differences): 2 values
+ * per node in 2D, 3 in 3D.  Returns -1 if the source is outside. */
+int ekxo_gradient2d(const ekxo_geometry2d *g, const double *slow,
+                    const double *T, double xs, double zs, double *grad);
+int ekxo_gradient3d(const ekxo_geometry3d *g, const double *slow,
+                    const double *T, double xs, double ys, double zs,
+                    double *grad);
 /* Bilinear travel time at a point (src/solver2d.cpp:106-167). */
 double ekxo_station_time2d(const ekxo_geometry2d *g, const double *slow,
                            const double *T, double xs, double zs, double xr,
