@@ -145,6 +145,19 @@ EXPORTED_SYMBOLS = [
     "ekx_solver2d_set_source", "ekx_solver2d_set_stations",
     "ekx_solver2d_solve", "ekx_solver2d_get_travel_times",
     "ekx_solver2d_get_travel_times_at_stations", "ekx_solver2d_get_stats",
+    "ekx_solver3d_set_stations", "ekx_solver3d_get_number_of_stations",
+    "ekx_solver3d_get_travel_times_at_stations",
+    "ekx_solver3d_compute_travel_time_gradient_field",
+    "ekx_solver3d_have_travel_time_gradient_field",
+    "ekx_solver3d_get_travel_time_gradient_field",
+    "ekx_solver3d_get_travel_time_gradient_field_device",
+    "ekx_batch3d_set_stations", "ekx_batch3d_get_travel_times_at_stations",
+    "ekx_batch3d_get_station_table",
+    "ekx_batch3d_compute_travel_time_gradient_field",
+    "ekx_batch3d_get_travel_time_gradient_field",
+    "ekx_solver2d_compute_travel_time_gradient_field",
+    "ekx_solver2d_have_travel_time_gradient_field",
+    "ekx_solver2d_get_travel_time_gradient_field",
 ]
 
 _lib = None
@@ -221,6 +234,30 @@ def _declare(L):
     L.ekx_solver2d_get_travel_times.argtypes = [_P, _I64, _PD]
     L.ekx_solver2d_get_travel_times_at_stations.argtypes = [_P, _I, _PD]
     L.ekx_solver2d_get_stats.argtypes = [_P, ctypes.POINTER(SolveStats)]
+    L.ekx_solver3d_set_stations.argtypes = [_P, _I, _PD, _PD, _PD]
+    L.ekx_solver3d_get_number_of_stations.argtypes = [
+        _P, ctypes.POINTER(ctypes.c_int)]
+    L.ekx_solver3d_get_travel_times_at_stations.argtypes = [_P, _I, _PD]
+    L.ekx_solver3d_compute_travel_time_gradient_field.argtypes = [_P]
+    L.ekx_solver3d_have_travel_time_gradient_field.argtypes = [_P]
+    L.ekx_solver3d_get_travel_time_gradient_field.argtypes = [_P, _I64, _PD]
+    L.ekx_solver3d_get_travel_time_gradient_field_device.argtypes = [
+        _P, ctypes.POINTER(_P)]
+    L.ekx_batch3d_set_stations.argtypes = [_P, _I, _PD, _PD, _PD]
+    L.ekx_batch3d_get_travel_times_at_stations.argtypes = [_P, _I, _I, _PD]
+    L.ekx_batch3d_get_station_table.argtypes = [_P, _I64, _PD]
+    L.ekx_batch3d_compute_travel_time_gradient_field.argtypes = [_P, _I]
+    L.ekx_batch3d_get_travel_time_gradient_field.argtypes = [_P, _I, _I64,
+                                                             _PD]
+    L.ekx_solver2d_compute_travel_time_gradient_field.argtypes = [_P]
+    L.ekx_solver2d_have_travel_time_gradient_field.argtypes = [_P]
+    L.ekx_solver2d_get_travel_time_gradient_field.argtypes = [_P, _I64, _PD]
+
+
+def _xyz(locations):
+    a = np.ascontiguousarray(locations, dtype=np.float64).reshape(-1, 3)
+    return (np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1]),
+            np.ascontiguousarray(a[:, 2]))
 
 
 def _check(status):
@@ -439,6 +476,49 @@ class Solver3D:
             self._h, ctypes.byref(ptr), ctypes.byref(size)))
         return ptr.value, size.value
 
+    def set_stations(self, locations):
+        """Solver3D::setStations (src/solver3d.cpp:555-609): absolute
+        (x, y, z) receiver locations; every solve interpolates its field at
+        them."""
+        x, y, z = _xyz(locations)
+        _check(self._L.ekx_solver3d_set_stations(self._h, x.size, _dp(x),
+                                                 _dp(y), _dp(z)))
+
+    def get_number_of_stations(self):
+        n = ctypes.c_int()
+        _check(self._L.ekx_solver3d_get_number_of_stations(self._h,
+                                                           ctypes.byref(n)))
+        return n.value
+
+    def get_travel_times_at_stations(self):
+        n = self.get_number_of_stations()
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_solver3d_get_travel_times_at_stations(
+            self._h, n, _dp(out)))
+        return out
+
+    def compute_travel_time_gradient_field(self):
+        _check(self._L.ekx_solver3d_compute_travel_time_gradient_field(
+            self._h))
+
+    def have_travel_time_gradient_field(self):
+        return bool(
+            self._L.ekx_solver3d_have_travel_time_gradient_field(self._h))
+
+    def get_travel_time_gradient_field(self):
+        """(dT/dx, dT/dy, dT/dz) per node, natural order: 3 n values."""
+        n = 3 * self._geometry.n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_solver3d_get_travel_time_gradient_field(
+            self._h, n, _dp(out)))
+        return out
+
+    def get_travel_time_gradient_field_device(self):
+        ptr = _P()
+        _check(self._L.ekx_solver3d_get_travel_time_gradient_field_device(
+            self._h, ctypes.byref(ptr)))
+        return ptr.value
+
     def get_stats(self):
         st = SolveStats()
         _check(self._L.ekx_solver3d_get_stats(self._h, ctypes.byref(st)))
@@ -461,6 +541,7 @@ class Batch3D:
         _check(self._L.ekx_batch3d_create(ctypes.byref(self._h)))
         self._keep = []
         self._geoms = []
+        self._n_stations = 0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -490,6 +571,7 @@ class Batch3D:
                                               len(problems), arr))
         self._keep = keep
         self._geoms = [p[0] for p in problems]
+        self._n_stations = 0
 
     def set_source(self, problem, location):
         x, y, z = location
@@ -504,6 +586,37 @@ class Batch3D:
         out = np.empty(n, dtype=np.float64)
         _check(self._L.ekx_batch3d_get_travel_times(self._h, int(problem), n,
                                                     _dp(out)))
+        return out
+
+    def set_stations(self, locations):
+        """Receivers shared by every problem: each solve fills a
+        problems x stations table of travel times."""
+        x, y, z = _xyz(locations)
+        _check(self._L.ekx_batch3d_set_stations(self._h, x.size, _dp(x),
+                                                _dp(y), _dp(z)))
+        self._n_stations = x.size
+
+    def get_travel_times_at_stations(self, problem):
+        n = self._n_stations
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_batch3d_get_travel_times_at_stations(
+            self._h, int(problem), n, _dp(out)))
+        return out
+
+    def get_station_table(self):
+        out = np.empty((len(self._geoms), self._n_stations),
+                       dtype=np.float64)
+        _check(self._L.ekx_batch3d_get_station_table(self._h, out.size,
+                                                     _dp(out)))
+        return out
+
+    def get_travel_time_gradient_field(self, problem):
+        _check(self._L.ekx_batch3d_compute_travel_time_gradient_field(
+            self._h, int(problem)))
+        n = 3 * self._geoms[problem].n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_batch3d_get_travel_time_gradient_field(
+            self._h, int(problem), n, _dp(out)))
         return out
 
     def get_stats(self):
@@ -574,6 +687,24 @@ class Solver2D:
         out = np.empty(self._n_stations, dtype=np.float64)
         _check(self._L.ekx_solver2d_get_travel_times_at_stations(
             self._h, self._n_stations, _dp(out)))
+        return out
+
+    def compute_travel_time_gradient_field(self):
+        """Solver2D::computeTravelTimeGradientField
+        (src/solver2d.cpp:866-879)."""
+        _check(self._L.ekx_solver2d_compute_travel_time_gradient_field(
+            self._h))
+
+    def have_travel_time_gradient_field(self):
+        return bool(
+            self._L.ekx_solver2d_have_travel_time_gradient_field(self._h))
+
+    def get_travel_time_gradient_field(self):
+        """(dT/dx, dT/dz) per node, natural order: 2 n values."""
+        n = 2 * self._geometry.n_grid
+        out = np.empty(n, dtype=np.float64)
+        _check(self._L.ekx_solver2d_get_travel_time_gradient_field(
+            self._h, n, _dp(out)))
         return out
 
     def get_stats(self):

@@ -13,6 +13,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 using namespace ekx;
@@ -179,7 +180,8 @@ ekx_status check_options(const ekx_solver_options *o)
 }
 
 // Source3D::setLocationIn{X,Y,Z} (src/source3d.cpp:161-291) and the range
-// checks of Solver3D::setSource (src/solver3d.cpp:299-325).
+// checks of Solver3D::setSource (src/solver3d.cpp:299-325); receivers are
+// located the same way (Station3D, src/station3d.cpp:158-291).
 struct SourceInfo
 {
     double offset[3];
@@ -188,26 +190,28 @@ struct SourceInfo
 };
 
 ekx_status locate_source3d(const ekx_geometry3d &g, double x, double y,
-                           double z, SourceInfo &s)
+                           double z, SourceInfo &s,
+                           const char *what = "source")
 {
+    const std::string pos = std::string(" ") + what + " position = ";
     const double xmax = g.x0 + (g.nx - 1)*g.dx;
     const double ymax = g.y0 + (g.ny - 1)*g.dy;
     const double zmax = g.z0 + (g.nz - 1)*g.dz;
     if (x < g.x0 || x > xmax)
     {
-        return fail(EKX_EINVAL, "x source position = " + fmt(x) +
+        return fail(EKX_EINVAL, "x" + pos + fmt(x) +
                                     " must be in range [" + fmt(g.x0) + "," +
                                     fmt(xmax) + "]");
     }
     if (y < g.y0 || y > ymax)
     {
-        return fail(EKX_EINVAL, "y source position = " + fmt(y) +
+        return fail(EKX_EINVAL, "y" + pos + fmt(y) +
                                     " must be in range [" + fmt(g.y0) + "," +
                                     fmt(ymax) + "]");
     }
     if (z < g.z0 || z > zmax)
     {
-        return fail(EKX_EINVAL, "z source position = " + fmt(z) +
+        return fail(EKX_EINVAL, "z" + pos + fmt(z) +
                                     " must be in range [" + fmt(g.z0) + "," +
                                     fmt(zmax) + "]");
     }
@@ -393,6 +397,21 @@ struct Engine3D
     bool haveField = false;
     bool fieldIsF64 = true;
     ekx_solve_stats stats{};
+    // Count of completed solves: derived results record the solve they
+    // were computed from.
+    uint64_t solveStamp = 0;
+    // Receivers: nStations per problem, located in every problem's grid
+    // (dStations is [problem][station]); their travel times after the
+    // solve numbered stationStamp.
+    int nStations = 0;
+    DeviceBuffer<StationLoc3D> dStations;
+    DeviceBuffer<double> stationOut;
+    std::vector<double> stationTimes;
+    uint64_t stationStamp = 0;
+    // Travel-time gradient (3 values per node) of one problem's field.
+    DeviceBuffer<double> G;
+    int gradientProblem = -1;
+    uint64_t gradientStamp = 0;
 
     ~Engine3D()
     {
@@ -465,6 +484,9 @@ struct Engine3D
         haveModel = false;
         haveField = false;
         stats = ekx_solve_stats{};
+        nStations = 0;
+        stationTimes.clear();
+        gradientProblem = -1;
         if (!stream)
         {
             EKX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1157,6 +1179,156 @@ struct Engine3D
         }
         stats = st;
         haveField = true;
+        solveStamp += 1;
+        if (nStations > 0) { EKX_CHECK(compute_stations()); }
+        return EKX_OK;
+    }
+
+    // Runs f(T) with the device field in the type it is stored in.
+    template <typename F> ekx_status with_field(F &&f) const
+    {
+        if (fieldIsF64) { return f(T64.ptr); }
+        if (TQ.ptr) { return f((const uint32_t *) TQ.ptr); }
+        return f((const float *) T32.ptr);
+    }
+
+    // Locates the receivers in every problem's grid.
+    ekx_status set_stations(int n, const double *x, const double *y,
+                            const double *z)
+    {
+        if (n < 0 || (n > 0 && (!x || !y || !z)))
+        {
+            return fail(EKX_EINVAL, "invalid station arrays");
+        }
+        const size_t np = problems.size();
+        std::vector<StationLoc3D> locs(np*(size_t) n);
+        for (size_t k = 0; k < np; ++k)
+        {
+            for (int i = 0; i < n; ++i)
+            {
+                SourceInfo s;
+                EKX_CHECK(locate_source3d(geometry[k], x[i], y[i], z[i], s,
+                                          "station"));
+                StationLoc3D &l = locs[k*(size_t) n + i];
+                for (int a = 0; a < 3; ++a)
+                {
+                    l.off[a] = s.offset[a];
+                    l.cell[a] = s.cell[a];
+                }
+                l.pad = 0;
+            }
+        }
+        nStations = 0;
+        stationTimes.clear();
+        if (n > 0)
+        {
+            EKX_CHECK(dStations.allocate(locs.size()));
+            EKX_CHECK(stationOut.allocate(locs.size()));
+            EKX_HIP(hipMemcpy(dStations.ptr, locs.data(),
+                              locs.size()*sizeof(StationLoc3D),
+                              hipMemcpyHostToDevice));
+        }
+        nStations = n;
+        if (haveField && n > 0) { EKX_CHECK(compute_stations()); }
+        return EKX_OK;
+    }
+
+    // Interpolates the solved fields at the receivers and copies the
+    // [problem][station] table to the host.
+    ekx_status compute_stations()
+    {
+        const int np = (int) problems.size();
+        EKX_CHECK(with_field([&](auto *T) -> ekx_status {
+            using TS = std::remove_cv_t<std::remove_pointer_t<
+                decltype(T)>>;
+            EKX_HIP(launch_stations3d<TS>(dProblems.ptr, np, dStations.ptr,
+                                          nStations, T, stationOut.ptr,
+                                          stream));
+            return EKX_OK;
+        }));
+        stationTimes.assign((size_t) np*nStations, 0.0);
+        EKX_HIP(hipMemcpyAsync(stationTimes.data(), stationOut.ptr,
+                               stationTimes.size()*sizeof(double),
+                               hipMemcpyDeviceToHost, stream));
+        EKX_HIP(hipStreamSynchronize(stream));
+        stationStamp = solveStamp;
+        return EKX_OK;
+    }
+
+    ekx_status copy_station_times(int problem, int n, double *out) const
+    {
+        if (!haveField || stationStamp != solveStamp)
+        {
+            return fail(EKX_ESTATE, "Travel time field not yet computed");
+        }
+        if (n != nStations || (n > 0 && !out))
+        {
+            return fail(EKX_EINVAL, "output must hold one value per "
+                                    "station (" +
+                                        std::to_string(nStations) + ")");
+        }
+        const double *src = stationTimes.data() + (size_t) problem*n;
+        std::copy(src, src + n, out);
+        return EKX_OK;
+    }
+
+    // Travel-time gradient of one problem into G (src/gradient.hpp).
+    ekx_status compute_gradient(int problem)
+    {
+        if (!haveField)
+        {
+            return fail(EKX_ESTATE, "Travel time field not yet computed");
+        }
+        if (gradientProblem == problem && gradientStamp == solveStamp)
+        {
+            return EKX_OK;
+        }
+        const ekx_geometry3d &g = geometry[problem];
+        const size_t nGrid = (size_t) g.nx*g.ny*g.nz;
+        size_t need = 0;
+        for (const ekx_geometry3d &q : geometry)
+        {
+            need = std::max(need, (size_t) q.nx*q.ny*q.nz);
+        }
+        gradientProblem = -1;
+        EKX_CHECK(G.allocate(3*std::max(need, nGrid)));
+        EKX_CHECK(with_field([&](auto *T) -> ekx_status {
+            using TS = std::remove_cv_t<std::remove_pointer_t<
+                decltype(T)>>;
+            EKX_HIP(launch_gradient3d<TS>(dProblems.ptr, problem,
+                                          problems[problem], T, G.ptr,
+                                          stream));
+            return EKX_OK;
+        }));
+        EKX_HIP(hipStreamSynchronize(stream));
+        gradientProblem = problem;
+        gradientStamp = solveStamp;
+        return EKX_OK;
+    }
+
+    bool have_gradient(int problem) const
+    {
+        return haveField && gradientProblem == problem &&
+               gradientStamp == solveStamp;
+    }
+
+    ekx_status copy_gradient(int problem, int64_t n, double *out) const
+    {
+        if (!have_gradient(problem))
+        {
+            return fail(EKX_ESTATE, "Travel time gradient field not yet "
+                                    "computed");
+        }
+        const ekx_geometry3d &g = geometry[problem];
+        const int64_t nOut = 3*(int64_t) g.nx*g.ny*g.nz;
+        if (n != nOut || !out)
+        {
+            return fail(EKX_EINVAL, "output must hold 3 x " +
+                                        std::to_string(nOut/3) +
+                                        " values");
+        }
+        EKX_HIP(hipMemcpy(out, G.ptr, (size_t) n*sizeof(double),
+                          hipMemcpyDeviceToHost));
         return EKX_OK;
     }
 
@@ -1508,6 +1680,62 @@ ekx_status ekx_solver3d_get_travel_times_device(const ekx_solver3d *h,
     return EKX_OK;
 }
 
+ekx_status ekx_solver3d_set_stations(ekx_solver3d *h, int n,
+                                     const double *x, const double *y,
+                                     const double *z)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.set_stations(n, x, y, z);
+}
+
+ekx_status ekx_solver3d_get_number_of_stations(const ekx_solver3d *h,
+                                               int *n)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!n) { return fail(EKX_EINVAL, "n is NULL"); }
+    *n = h->engine.nStations;
+    return EKX_OK;
+}
+
+ekx_status ekx_solver3d_get_travel_times_at_stations(const ekx_solver3d *h,
+                                                     int n, double *out)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.copy_station_times(0, n, out);
+}
+
+ekx_status ekx_solver3d_compute_travel_time_gradient_field(ekx_solver3d *h)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.compute_gradient(0);
+}
+
+int ekx_solver3d_have_travel_time_gradient_field(const ekx_solver3d *h)
+{
+    return h && h->initialized && h->engine.have_gradient(0) ? 1 : 0;
+}
+
+ekx_status ekx_solver3d_get_travel_time_gradient_field(
+    const ekx_solver3d *h, int64_t n, double *out)
+{
+    EKX_CHECK(require_initialized(h));
+    return h->engine.copy_gradient(0, n, out);
+}
+
+ekx_status ekx_solver3d_get_travel_time_gradient_field_device(
+    const ekx_solver3d *h, const double **ptr)
+{
+    EKX_CHECK(require_initialized(h));
+    if (!h->engine.have_gradient(0))
+    {
+        return fail(EKX_ESTATE, "Travel time gradient field not yet "
+                                "computed");
+    }
+    if (!ptr) { return fail(EKX_EINVAL, "ptr is NULL"); }
+    *ptr = h->engine.G.ptr;
+    return EKX_OK;
+}
+
 ekx_status ekx_solver3d_get_stats(const ekx_solver3d *h,
                                   ekx_solve_stats *stats)
 {
@@ -1648,6 +1876,67 @@ ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *h, int problem,
         return fail(EKX_EINVAL, "problem index out of range");
     }
     return h->engine.copy_field(problem, n, out);
+}
+
+static ekx_status check_problem(const ekx_batch3d *h, int problem)
+{
+    if (problem < 0 || problem >= (int) h->engine.problems.size())
+    {
+        return fail(EKX_EINVAL, "problem index out of range");
+    }
+    return EKX_OK;
+}
+
+ekx_status ekx_batch3d_set_stations(ekx_batch3d *h, int n, const double *x,
+                                    const double *y, const double *z)
+{
+    EKX_CHECK(require_batch(h));
+    return h->engine.set_stations(n, x, y, z);
+}
+
+ekx_status ekx_batch3d_get_travel_times_at_stations(const ekx_batch3d *h,
+                                                    int problem, int n,
+                                                    double *out)
+{
+    EKX_CHECK(require_batch(h));
+    EKX_CHECK(check_problem(h, problem));
+    return h->engine.copy_station_times(problem, n, out);
+}
+
+ekx_status ekx_batch3d_get_station_table(const ekx_batch3d *h, int64_t n,
+                                         double *out)
+{
+    EKX_CHECK(require_batch(h));
+    const Engine3D &e = h->engine;
+    const int64_t want = (int64_t) e.problems.size()*e.nStations;
+    if (!e.haveField || e.stationStamp != e.solveStamp)
+    {
+        return fail(EKX_ESTATE, "Travel time field not yet computed");
+    }
+    if (n != want || (n > 0 && !out))
+    {
+        return fail(EKX_EINVAL, "output must hold problems x stations = " +
+                                    std::to_string(want) + " values");
+    }
+    std::copy(e.stationTimes.begin(), e.stationTimes.end(), out);
+    return EKX_OK;
+}
+
+ekx_status ekx_batch3d_compute_travel_time_gradient_field(ekx_batch3d *h,
+                                                          int problem)
+{
+    EKX_CHECK(require_batch(h));
+    EKX_CHECK(check_problem(h, problem));
+    return h->engine.compute_gradient(problem);
+}
+
+ekx_status ekx_batch3d_get_travel_time_gradient_field(const ekx_batch3d *h,
+                                                      int problem, int64_t n,
+                                                      double *out)
+{
+    EKX_CHECK(require_batch(h));
+    EKX_CHECK(check_problem(h, problem));
+    return h->engine.copy_gradient(problem, n, out);
 }
 
 ekx_status ekx_batch3d_get_stats(const ekx_batch3d *h,

@@ -125,6 +125,9 @@ struct ekx_solver2d
     bool initialized = false, haveModel = false, haveSource = false;
     bool haveField = false;
     std::vector<double> stationTimes;
+    // travel-time gradient (2 values per node) of the current field
+    Buf<double> G;
+    bool haveGradient = false;
     ekx_solve_stats stats{};
     ~ekx_solver2d()
     {
@@ -401,6 +404,7 @@ ekx_status ekx_solver2d_solve(ekx_solver2d *h)
     st.last_solve_ms = ms;
     h->stats = st;
     h->haveField = true;
+    h->haveGradient = false;
     return EKX_OK;
 }
 
@@ -432,6 +436,55 @@ ekx_status ekx_solver2d_get_travel_times_at_stations(const ekx_solver2d *h,
         return fail2(EKX_EINVAL, "output must hold one value per station");
     }
     for (int i = 0; i < n; ++i) { out[i] = h->stationTimes[(size_t) i]; }
+    return EKX_OK;
+}
+
+// Solver2D::computeTravelTimeGradientField (src/solver2d.cpp:866-879).
+ekx_status ekx_solver2d_compute_travel_time_gradient_field(ekx_solver2d *h)
+{
+    EKX2_CHECK(require2d(h));
+    if (!h->haveField)
+    {
+        return fail2(EKX_ESTATE, "Travel time field not yet computed");
+    }
+    if (h->haveGradient) { return EKX_OK; }
+    const ekx_geometry2d &g = h->geometry;
+    if (g.nx < 3 || g.nz < 3)
+    {
+        return fail2(EKX_EINVAL, "At least 3 grid points in x and z for "
+                                 "central difference");
+    }
+    EKX2_CHECK(h->G.allocate(2*(size_t) g.nx*g.nz));
+    EKX2_HIP(launch_gradient2d(h->dProblem.ptr, g.nx, g.nz, h->T.ptr,
+                               h->G.ptr, h->stream));
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    h->haveGradient = true;
+    return EKX_OK;
+}
+
+int ekx_solver2d_have_travel_time_gradient_field(const ekx_solver2d *h)
+{
+    return h && h->initialized && h->haveField && h->haveGradient ? 1 : 0;
+}
+
+ekx_status ekx_solver2d_get_travel_time_gradient_field(const ekx_solver2d *h,
+                                                       int64_t n,
+                                                       double *out)
+{
+    EKX2_CHECK(require2d(h));
+    if (!h->haveField || !h->haveGradient)
+    {
+        return fail2(EKX_ESTATE, "Travel time gradient field not yet "
+                                 "computed");
+    }
+    const int64_t want = 2*(int64_t) h->geometry.nx*h->geometry.nz;
+    if (n != want || !out)
+    {
+        return fail2(EKX_EINVAL, "output must hold 2 x " +
+                                     std::to_string(want/2) + " values");
+    }
+    EKX2_HIP(hipMemcpy(out, h->G.ptr, (size_t) n*sizeof(double),
+                       hipMemcpyDeviceToHost));
     return EKX_OK;
 }
 

@@ -57,6 +57,15 @@ struct ColumnItem
     int bx, by, bz;             // column and chunk index, permuted frame
 };
 
+// A receiver located in one problem's grid (Station3D, src/station3d.cpp:
+// 158-291): offset from the model origin and the cell that holds it.
+struct StationLoc3D
+{
+    double off[3];
+    int cell[3];
+    int pad;
+};
+
 // One problem of a 2D batch.
 struct Problem2D
 {

@@ -1,0 +1,244 @@
+// ekx_fields.hip -- quantities derived from a solved travel-time field:
+// travel times at receivers (stations) and travel-time gradients.
+//
+// Both are elementwise over HBM-resident fields: a station reads the 8
+// corners of its cell, a gradient node reads its 6 axis neighbours (12 at
+// the edges) and writes 2 or 3 FP64 components.  Neighbour reads hit in
+// L2, so a gradient launch moves about 8 B (FP64 field) + 24 B (output) per
+// node.  Arithmetic follows the oracle's operation order so that FP64
+// fields give bit-identical results.
+#include "ekx_kernels.hpp"
+#include "ekx_storage.hpp"
+
+namespace ekx
+{
+
+namespace
+{
+
+template <typename TS>
+__device__ __forceinline__ double field_at(const TS *Tp, int64_t i,
+                                           const Problem3D &p)
+{
+    return (double) Store<TS>::time(Tp[i], p);
+}
+
+// Second-order derivative along one axis of stride s at index position
+// idx of n: central inside, the one-sided 3-point formulas on the edges
+// (src/gradient.hpp:100-181).
+template <typename TS>
+__device__ __forceinline__ double axis_derivative(const TS *Tp, int64_t i,
+                                                  int64_t s, int idx, int n,
+                                                  double u, double twoInv,
+                                                  const Problem3D &p)
+{
+    if (idx == 0)
+    {
+        return (-3*u + 4*field_at(Tp, i + s, p) - field_at(Tp, i + 2*s, p))*
+               twoInv;
+    }
+    if (idx == n - 1)
+    {
+        return (field_at(Tp, i - 2*s, p) - 4*field_at(Tp, i - s, p) + 3*u)*
+               twoInv;
+    }
+    return (field_at(Tp, i + s, p) - field_at(Tp, i - s, p))*twoInv;
+}
+
+} // namespace
+
+// Trilinear travel time at each station of each problem: the 2D rule of
+// Solver2DImpl::interpolateTravelTimes (src/solver2d.cpp:106-167, with
+// src/ray/bilinear.hpp) on three axes -- the analytic time of the source
+// cell's slowness for a receiver in the source cell, else interpolation
+// of the cell's 8 corners in x, then y, then z.  Block (x) covers 256
+// stations, block (y) is the problem; out is [problem][station].
+template <typename TS>
+__global__ void __launch_bounds__(256)
+stations3d_kernel(const Problem3D *__restrict__ problems,
+                  const StationLoc3D *__restrict__ locs, int nStations,
+                  const TS *__restrict__ T, double *__restrict__ out)
+{
+    const int i = blockIdx.x*blockDim.x + threadIdx.x;
+    if (i >= nStations) { return; }
+    const int k = blockIdx.y;
+    const Problem3D &p = problems[k];
+    const size_t slot = (size_t) k*nStations + i;
+    const StationLoc3D l = locs[slot];
+    const double xi = l.off[0], yi = l.off[1], zi = l.off[2];
+    if (l.cell[0] == p.srcCellX && l.cell[1] == p.srcCellY &&
+        l.cell[2] == p.srcCellZ)
+    {
+        const double ddx = xi - p.srcOffX;
+        const double ddy = yi - p.srcOffY;
+        const double ddz = zi - p.srcOffZ;
+        out[slot] = sqrt(ddx*ddx + ddy*ddy + ddz*ddz)*p.srcSlowness;
+        return;
+    }
+    const TS *Tp = T + p.tOffset;
+    const int64_t sy = p.nx, sz = (int64_t) p.nx*p.ny;
+    const int64_t i000 = ((int64_t) l.cell[2]*p.ny + l.cell[1])*p.nx +
+                         l.cell[0];
+    const double dxi = 1./p.dx, dyi = 1./p.dy, dzi = 1./p.dz;
+    const double x0 = l.cell[0]*p.dx, x1 = (l.cell[0] + 1)*p.dx;
+    const double y0 = l.cell[1]*p.dy, y1 = (l.cell[1] + 1)*p.dy;
+    const double z0 = l.cell[2]*p.dz, z1 = (l.cell[2] + 1)*p.dz;
+    const double x2mx = x1 - xi, xmx1 = xi - x0;
+    const double y2my = y1 - yi, ymy1 = yi - y0;
+    const double z2mz = z1 - zi, zmz1 = zi - z0;
+    double f[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+    {
+        const int64_t b = i000 + c*sz;
+        const double fy0 = dxi*(x2mx*field_at(Tp, b, p) +
+                                xmx1*field_at(Tp, b + 1, p));
+        const double fy1 = dxi*(x2mx*field_at(Tp, b + sy, p) +
+                                xmx1*field_at(Tp, b + sy + 1, p));
+        f[c] = dyi*(y2my*fy0 + ymy1*fy1);
+    }
+    out[slot] = dzi*(z2mz*f[0] + zmz1*f[1]);
+}
+
+// Travel-time gradient of problem k (src/gradient.hpp:19-260, central
+// differences, on three axes), (dT/dx, dT/dy, dT/dz) per node in natural
+// order.  When the source is off the lower corner of its cell, the 8
+// corners of that cell take the analytic gradient of the source cell's
+// slowness along the axes the source is off.  Block: 64 nodes in x by 4
+// in y; grid: (x blocks, y blocks, nz).
+template <typename TS>
+__global__ void __launch_bounds__(256)
+gradient3d_kernel(const Problem3D *__restrict__ problems, int k,
+                  const TS *__restrict__ T, double *__restrict__ grad)
+{
+    const Problem3D &p = problems[k];
+    const int ix = blockIdx.x*64 + (int) (threadIdx.x & 63);
+    const int iy = blockIdx.y*4 + (int) (threadIdx.x >> 6);
+    const int iz = blockIdx.z;
+    if (ix >= p.nx || iy >= p.ny) { return; }
+    const TS *Tp = T + p.tOffset;
+    const int64_t sy = p.nx, sz = (int64_t) p.nx*p.ny;
+    const int64_t i = (int64_t) iz*sz + (int64_t) iy*sy + ix;
+    const double u = field_at(Tp, i, p);
+    double g[3];
+    g[0] = axis_derivative(Tp, i, 1, ix, p.nx, u, 1./(2.*p.dx), p);
+    g[1] = axis_derivative(Tp, i, sy, iy, p.ny, u, 1./(2.*p.dy), p);
+    g[2] = axis_derivative(Tp, i, sz, iz, p.nz, u, 1./(2.*p.dz), p);
+    const int ox = ix - p.srcCellX, oy = iy - p.srcCellY;
+    const int oz = iz - p.srcCellZ;
+    if ((unsigned) ox <= 1u && (unsigned) oy <= 1u && (unsigned) oz <= 1u)
+    {
+        const double tol = 1.e-7;
+        const bool doX = fabs((double) p.srcCellX*p.dx - p.srcOffX) > tol;
+        const bool doY = fabs((double) p.srcCellY*p.dy - p.srcOffY) > tol;
+        const bool doZ = fabs((double) p.srcCellZ*p.dz - p.srcOffZ) > tol;
+        if (doX || doY || doZ)
+        {
+            const double ddx = (p.srcCellX + ox)*p.dx - p.srcOffX;
+            const double ddy = (p.srcCellY + oy)*p.dy - p.srcOffY;
+            const double ddz = (p.srcCellZ + oz)*p.dz - p.srcOffZ;
+            const double r = sqrt(ddx*ddx + ddy*ddy + ddz*ddz);
+            const double s0 = p.srcSlowness;
+            if (doX) { g[0] = (ddx/r)*s0; }
+            if (doY) { g[1] = (ddy/r)*s0; }
+            if (doZ) { g[2] = (ddz/r)*s0; }
+        }
+    }
+    double *o = grad + 3*i;
+    o[0] = g[0];
+    o[1] = g[1];
+    o[2] = g[2];
+}
+
+// The 2D gradient (src/gradient.hpp:19-260), (dT/dx, dT/dz) per node.
+// Block: 64 nodes in x by 4 in z.
+__global__ void __launch_bounds__(256)
+gradient2d_kernel(const Problem2D *__restrict__ problem,
+                  const double *__restrict__ T, double *__restrict__ grad)
+{
+    const Problem2D &p = *problem;
+    const int ix = blockIdx.x*64 + (int) (threadIdx.x & 63);
+    const int iz = blockIdx.y*4 + (int) (threadIdx.x >> 6);
+    if (ix >= p.nx || iz >= p.nz) { return; }
+    const double *Tp = T + p.tOffset;
+    const int nx = p.nx, nz = p.nz;
+    const int64_t i = (int64_t) iz*nx + ix;
+    const double u = Tp[i];
+    const double twodxi = 1./(2.*p.dx), twodzi = 1./(2.*p.dz);
+    double gx, gz;
+    if (ix == 0) { gx = (-3*u + 4*Tp[i + 1] - Tp[i + 2])*twodxi; }
+    else if (ix == nx - 1) { gx = (Tp[i - 2] - 4*Tp[i - 1] + 3*u)*twodxi; }
+    else { gx = (Tp[i + 1] - Tp[i - 1])*twodxi; }
+    if (iz == 0) { gz = (-3*u + 4*Tp[i + nx] - Tp[i + 2*nx])*twodzi; }
+    else if (iz == nz - 1)
+    {
+        gz = (Tp[i - 2*nx] - 4*Tp[i - nx] + 3*u)*twodzi;
+    }
+    else { gz = (Tp[i + nx] - Tp[i - nx])*twodzi; }
+    const int ox = ix - p.srcCellX, oz = iz - p.srcCellZ;
+    if ((unsigned) ox <= 1u && (unsigned) oz <= 1u)
+    {
+        const double tol = 1.e-7;
+        const bool doX = fabs((double) p.srcCellX*p.dx - p.srcOffX) > tol;
+        const bool doZ = fabs((double) p.srcCellZ*p.dz - p.srcOffZ) > tol;
+        if (doX || doZ)
+        {
+            const double ddx = (p.srcCellX + ox)*p.dx - p.srcOffX;
+            const double ddz = (p.srcCellZ + oz)*p.dz - p.srcOffZ;
+            const double r = sqrt(ddx*ddx + ddz*ddz);
+            if (doX) { gx = (ddx/r)*p.srcSlowness; }
+            if (doZ) { gz = (ddz/r)*p.srcSlowness; }
+        }
+    }
+    grad[2*i] = gx;
+    grad[2*i + 1] = gz;
+}
+
+template <typename TS>
+hipError_t launch_stations3d(const Problem3D *dProblems, int nProblems,
+                             const StationLoc3D *dLocs, int nStations,
+                             const TS *T, double *out, hipStream_t stream)
+{
+    if (nStations <= 0 || nProblems <= 0) { return hipSuccess; }
+    dim3 grid((unsigned) ((nStations + 255)/256), (unsigned) nProblems);
+    hipLaunchKernelGGL(stations3d_kernel<TS>, grid, dim3(256), 0, stream,
+                       dProblems, dLocs, nStations, T, out);
+    return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t launch_gradient3d(const Problem3D *dProblems, int problem,
+                             const Problem3D &host, const TS *T,
+                             double *grad, hipStream_t stream)
+{
+    dim3 grid((unsigned) ((host.nx + 63)/64), (unsigned) ((host.ny + 3)/4),
+              (unsigned) host.nz);
+    hipLaunchKernelGGL(gradient3d_kernel<TS>, grid, dim3(256), 0, stream,
+                       dProblems, problem, T, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_gradient2d(const Problem2D *dProblem, int nx, int nz,
+                             const double *T, double *grad,
+                             hipStream_t stream)
+{
+    dim3 grid((unsigned) ((nx + 63)/64), (unsigned) ((nz + 3)/4));
+    hipLaunchKernelGGL(gradient2d_kernel, grid, dim3(256), 0, stream,
+                       dProblem, T, grad);
+    return hipGetLastError();
+}
+
+#define EKX_FIELDS_INSTANTIATE(TS)                                          \
+    template hipError_t launch_stations3d<TS>(const Problem3D *, int,        \
+                                              const StationLoc3D *, int,    \
+                                              const TS *, double *,         \
+                                              hipStream_t);                 \
+    template hipError_t launch_gradient3d<TS>(const Problem3D *, int,        \
+                                              const Problem3D &, const TS *, \
+                                              double *, hipStream_t);
+EKX_FIELDS_INSTANTIATE(float)
+EKX_FIELDS_INSTANTIATE(double)
+EKX_FIELDS_INSTANTIATE(uint32_t)
+#undef EKX_FIELDS_INSTANTIATE
+
+} // namespace ekx

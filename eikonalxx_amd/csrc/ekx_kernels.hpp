@@ -69,6 +69,22 @@ hipError_t launch_convert(int64_t n, const double *in, float *out,
 hipError_t launch_convert(int64_t n, const float *in, double *out,
                           hipStream_t stream);
 
+// Travel times at stations: out[k*nStations + i] for problem k, station i
+// (locations per problem, same layout).
+template <typename TS>
+hipError_t launch_stations3d(const Problem3D *dProblems, int nProblems,
+                             const StationLoc3D *dLocs, int nStations,
+                             const TS *T, double *out, hipStream_t stream);
+// Travel-time gradient of one problem: 3 values per node (host: the same
+// problem's descriptor, for the grid size).
+template <typename TS>
+hipError_t launch_gradient3d(const Problem3D *dProblems, int problem,
+                             const Problem3D &host, const TS *T,
+                             double *grad, hipStream_t stream);
+hipError_t launch_gradient2d(const Problem2D *dProblem, int nx, int nz,
+                             const double *T, double *grad,
+                             hipStream_t stream);
+
 hipError_t launch_solve2d(const Problem2D *dProblems, int nProblems,
                           double *T, const double *S, hipStream_t stream);
 hipError_t launch_stations2d(const Problem2D *dProblem, int n,

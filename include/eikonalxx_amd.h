@@ -246,6 +246,42 @@ EKX_API ekx_status ekx_solver3d_get_travel_times_f32(
 EKX_API ekx_status ekx_solver3d_get_travel_times_device(
     const ekx_solver3d *solver, const void **device_ptr,
     int *element_size);
+/* Solver3D::setStations (solver3d.hpp:113, src/solver3d.cpp:555-609):
+ * receiver locations, absolute x, y and z, each inside the model (the
+ * checks of Station3D::setLocationIn*, src/station3d.cpp:158-291).  The
+ * reference only stores 3D stations; here every solve also interpolates
+ * its field at them (as Solver2D does, src/solver2d.cpp:106-167, on three
+ * axes), and a call after a solve interpolates the current field. */
+EKX_API ekx_status ekx_solver3d_set_stations(ekx_solver3d *solver, int n,
+                                             const double *x,
+                                             const double *y,
+                                             const double *z);
+/* Solver3D::getStations().size(). */
+EKX_API ekx_status ekx_solver3d_get_number_of_stations(
+    const ekx_solver3d *solver, int *n);
+/* The 3D counterpart of Solver2D::getTravelTimesAtStations
+ * (solver2d.hpp:148): trilinear interpolation of the field at each
+ * station, the analytic time of the source cell's slowness for a station
+ * in the source cell.  n must equal the number of stations. */
+EKX_API ekx_status ekx_solver3d_get_travel_times_at_stations(
+    const ekx_solver3d *solver, int n, double *out);
+/* The 3D counterpart of Solver2D::computeTravelTimeGradientField
+ * (solver2d.hpp:127, src/solver2d.cpp:866-879, src/gradient.hpp): central
+ * differences inside, second-order one-sided ones on the faces, the
+ * analytic gradient at the corners of the source cell. */
+EKX_API ekx_status ekx_solver3d_compute_travel_time_gradient_field(
+    ekx_solver3d *solver);
+/* Solver3D::haveTravelTimeGradientField (solver3d.hpp:163). */
+EKX_API int ekx_solver3d_have_travel_time_gradient_field(
+    const ekx_solver3d *solver);
+/* Solver3D::getTravelTimeGradientField (solver3d.hpp:153): 3 x nx*ny*nz
+ * values, (dT/dx, dT/dy, dT/dz) per node in natural order. */
+EKX_API ekx_status ekx_solver3d_get_travel_time_gradient_field(
+    const ekx_solver3d *solver, int64_t n, double *out);
+/* Solver3D::getTravelTimeGradientFieldPointer (solver3d.hpp:160), as a
+ * device pointer borrowed until the next solve or gradient call. */
+EKX_API ekx_status ekx_solver3d_get_travel_time_gradient_field_device(
+    const ekx_solver3d *solver, const double **device_ptr);
 EKX_API ekx_status ekx_solver3d_get_stats(const ekx_solver3d *solver,
                                           ekx_solve_stats *stats);
 /* The handle's HIP stream (hipStream_t), for callers that time or order
@@ -280,6 +316,24 @@ EKX_API ekx_status ekx_batch3d_solve(ekx_batch3d *batch);
 EKX_API ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *batch,
                                                 int problem, int64_t n,
                                                 double *out);
+/* Receivers shared by every problem (each must lie inside every problem's
+ * model): after each solve the batch holds a problems x stations table of
+ * travel times, e.g. the station-reciprocal tables of a locator when the
+ * problems' sources are the stations. */
+EKX_API ekx_status ekx_batch3d_set_stations(ekx_batch3d *batch, int n,
+                                            const double *x,
+                                            const double *y,
+                                            const double *z);
+EKX_API ekx_status ekx_batch3d_get_travel_times_at_stations(
+    const ekx_batch3d *batch, int problem, int n, double *out);
+/* The whole table, [problem][station]; n = problems x stations. */
+EKX_API ekx_status ekx_batch3d_get_station_table(const ekx_batch3d *batch,
+                                                 int64_t n, double *out);
+/* Gradient of one problem's field (one problem held at a time). */
+EKX_API ekx_status ekx_batch3d_compute_travel_time_gradient_field(
+    ekx_batch3d *batch, int problem);
+EKX_API ekx_status ekx_batch3d_get_travel_time_gradient_field(
+    const ekx_batch3d *batch, int problem, int64_t n, double *out);
 EKX_API ekx_status ekx_batch3d_get_stats(const ekx_batch3d *batch,
                                          ekx_solve_stats *stats);
 EKX_API ekx_status ekx_batch3d_get_stream(const ekx_batch3d *batch,
@@ -324,6 +378,17 @@ EKX_API ekx_status ekx_solver2d_get_travel_times(const ekx_solver2d *solver,
 /* Solver2D::getTravelTimesAtStations. */
 EKX_API ekx_status ekx_solver2d_get_travel_times_at_stations(
     const ekx_solver2d *solver, int n, double *out);
+/* Solver2D::computeTravelTimeGradientField (solver2d.hpp:127,
+ * src/solver2d.cpp:866-879, src/gradient.hpp). */
+EKX_API ekx_status ekx_solver2d_compute_travel_time_gradient_field(
+    ekx_solver2d *solver);
+/* Solver2D::haveTravelTimeGradientField (src/solver2d.cpp:919-922). */
+EKX_API int ekx_solver2d_have_travel_time_gradient_field(
+    const ekx_solver2d *solver);
+/* Solver2D::getTravelTimeGradientField (src/solver2d.cpp:925-932):
+ * 2 x nx*nz values, (dT/dx, dT/dz) per node. */
+EKX_API ekx_status ekx_solver2d_get_travel_time_gradient_field(
+    const ekx_solver2d *solver, int64_t n, double *out);
 EKX_API ekx_status ekx_solver2d_get_stats(const ekx_solver2d *solver,
                                           ekx_solve_stats *stats);
 

@@ -15,12 +15,17 @@
 #pragma once
 #include "eikonalxx_amd.h"
 
+#include <cctype>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <limits>
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -490,6 +495,368 @@ using Station2D = Source2D;
 using Station3D = Source3D;
 
 // ---------------------------------------------------------------------------
+// IO::VTKRectilinearGrid3D / VTKRectilinearGrid2D (include/eikonalxx/io/
+// vtkRectilinearGrid{3d,2d}.hpp): legacy-VTK rectilinear grids laid out as
+// src/io/vtkRectilinearGrid3d.cpp and vtkRectilinearGrid2d.cpp write them
+// -- header, title with blanks as underscores, BINARY (big-endian 4-byte
+// floats, src/io/pack.hpp) or ASCII, coordinates in float arithmetic with
+// the depth axis negated, POINT_DATA once per file, LOOKUP_TABLE after the
+// first dataset only.  Data in XYZ (Ordering*::XYZ) order is transposed to
+// VTK's x-fastest order.
+// ---------------------------------------------------------------------------
+namespace IO
+{
+namespace detail
+{
+inline std::string blanks_to_underscores(const std::string &s)
+{
+    std::string r = s;
+    for (char &c : r)
+    {
+        if (std::isspace(static_cast<unsigned char>(c))) { c = '_'; }
+    }
+    return r;
+}
+
+inline void put_big_endian(float f, char *out)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    out[0] = static_cast<char>(u >> 24);
+    out[1] = static_cast<char>(u >> 16);
+    out[2] = static_cast<char>(u >> 8);
+    out[3] = static_cast<char>(u);
+}
+
+// Index of node/cell (ix, iy, iz) in natural (x fastest) and XYZ (z
+// fastest) order.
+inline size_t natural_index(int n1, int n2, int i, int j, int k)
+{
+    return ((size_t) k*n2 + j)*n1 + i;
+}
+
+// The state and the layout rules both grids share.
+class RectilinearWriter
+{
+public:
+    [[nodiscard]] bool isOpen() const noexcept { return mFile.is_open(); }
+    void close() noexcept
+    {
+        if (mFile.is_open()) { mFile.close(); }
+        mBinary = true;
+        mWrotePointData = false;
+        mWroteLookupTable = false;
+    }
+protected:
+    // Header and coordinates; coordinates[a] hold the axis' float values.
+    void start(const std::string &fileName, const std::string &title,
+               bool writeBinary, const int n[3],
+               const std::vector<float> coordinates[3],
+               const std::vector<float> asciiCoordinates[3])
+    {
+        close();
+        const std::filesystem::path path(fileName);
+        if (!std::filesystem::exists(path) && !path.parent_path().empty() &&
+            !std::filesystem::exists(path.parent_path()))
+        {
+            std::filesystem::create_directory(path.parent_path());
+        }
+        mBinary = writeBinary;
+        mFile.open(fileName, std::ios::out | std::ios::trunc |
+                                 (writeBinary ? std::ios::binary
+                                              : std::ios::openmode{}));
+        if (!mFile.is_open())
+        {
+            throw std::invalid_argument("could not open " + fileName);
+        }
+        mFile << "# vtk DataFile Version 3.0\n";
+        mFile << (title.empty() ? std::string("vtk_dataset")
+                                : blanks_to_underscores(title)) << "\n";
+        mFile << (writeBinary ? "BINARY" : "ASCII") << "\n";
+        mFile << "DATASET RECTILINEAR_GRID\n";
+        mFile << "DIMENSIONS " << n[0] << " " << n[1] << " " << n[2] << "\n";
+        const char *axis[3] = {"X_COORDINATES ", "Y_COORDINATES ",
+                               "Z_COORDINATES "};
+        for (int a = 0; a < 3; ++a)
+        {
+            mFile << axis[a] << n[a] << " float\n";
+            if (writeBinary)
+            {
+                write_floats(coordinates[a]);
+                mFile << "\n";
+            }
+            else
+            {
+                for (float v : asciiCoordinates[a]) { mFile << v << "\n"; }
+            }
+        }
+    }
+    // SCALARS block of natural-order values; point data announce
+    // POINT_DATA once, cell data announce CELL_DATA every time.
+    void scalars(bool point, const std::string &name,
+                 const std::vector<float> &v) const
+    {
+        require_open();
+        if (point)
+        {
+            if (!mWrotePointData)
+            {
+                mFile << "POINT_DATA " << v.size() << "\n";
+                mWrotePointData = true;
+            }
+        }
+        else
+        {
+            mFile << "CELL_DATA " << v.size() << "\n";
+        }
+        mFile << "SCALARS "
+              << (name.empty() ? std::string("dataset")
+                               : blanks_to_underscores(name))
+              << " float 1\n";
+        lookup_table();
+        if (mBinary) { write_floats(v); }
+        else
+        {
+            for (float x : v) { mFile << x << "\n"; }
+        }
+    }
+    // VECTORS block, 3 components per node (natural order).
+    void vectors(const std::string &name, size_t nGrid,
+                 const std::vector<float> &v3) const
+    {
+        require_open();
+        if (!mWrotePointData)
+        {
+            mFile << "POINT_DATA " << nGrid << "\n";
+            mWrotePointData = true;
+        }
+        mFile << "VECTORS "
+              << (name.empty() ? std::string("vector_dataset")
+                               : blanks_to_underscores(name))
+              << " float \n";
+        lookup_table();
+        if (mBinary) { write_floats(v3); }
+        else
+        {
+            for (size_t i = 0; i < nGrid; ++i)
+            {
+                mFile << v3[3*i] << " " << v3[3*i + 1] << " "
+                      << v3[3*i + 2] << "\n";
+            }
+        }
+    }
+private:
+    void require_open() const
+    {
+        if (!mFile.is_open())
+        {
+            throw std::runtime_error("File not open for writing");
+        }
+    }
+    void lookup_table() const
+    {
+        if (!mWroteLookupTable)
+        {
+            mFile << "LOOKUP_TABLE default\n";
+            mWroteLookupTable = true;
+        }
+    }
+    void write_floats(const std::vector<float> &v) const
+    {
+        std::vector<char> bytes(4*v.size());
+        for (size_t i = 0; i < v.size(); ++i)
+        {
+            put_big_endian(v[i], &bytes[4*i]);
+        }
+        mFile.write(bytes.data(), (std::streamsize) bytes.size());
+    }
+    mutable std::ofstream mFile;
+    mutable bool mWrotePointData = false, mWroteLookupTable = false;
+    bool mBinary = true;
+};
+} // namespace detail
+
+class VTKRectilinearGrid3D : public detail::RectilinearWriter
+{
+public:
+    ~VTKRectilinearGrid3D() { close(); }
+    void open(const std::string &fileName, const Geometry3D &geometry,
+              const std::string &title = "traveltimes",
+              bool writeBinary = true)
+    {
+        (void) geometry.c();
+        mGeometry = geometry;
+        const int n[3] = {geometry.getNumberOfGridPointsInX(),
+                          geometry.getNumberOfGridPointsInY(),
+                          geometry.getNumberOfGridPointsInZ()};
+        const float d[3] = {(float) geometry.getGridSpacingInX(),
+                            (float) geometry.getGridSpacingInY(),
+                            (float) geometry.getGridSpacingInZ()};
+        const float o[3] = {(float) geometry.getOriginInX(),
+                            (float) geometry.getOriginInY(),
+                            (float) geometry.getOriginInZ()};
+        std::vector<float> c[3], a[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            for (int i = 0; i < n[k]; ++i)
+            {
+                float v = o[k] + d[k]*i;
+                c[k].push_back(k == 2 ? -v : v);
+                a[k].push_back(k == 2 ? o[k] - d[k]*i : v);
+            }
+        }
+        start(fileName, title, writeBinary, n, c, a);
+    }
+    template <typename T>
+    void writeNodalDataset(const std::string &name, const T *data,
+                           Ordering3D ordering = Ordering3D::Natural) const
+    {
+        scalars(true, name, natural(data, 0, ordering));
+    }
+    template <typename T>
+    void writeCellularDataset(const std::string &name, const T *data,
+                              Ordering3D ordering = Ordering3D::Natural) const
+    {
+        scalars(false, name, natural(data, 1, ordering));
+    }
+    // (vx, vy, vz) per node in natural order -- an extension: the
+    // reference's 3D grid writes scalars only.
+    template <typename T>
+    void writeNodalVectorDataset(const std::string &name, const T *data) const
+    {
+        if (data == nullptr) { throw std::invalid_argument("data is NULL"); }
+        const size_t n = (size_t) mGeometry.getNumberOfGridPoints();
+        std::vector<float> v(data, data + 3*n);
+        vectors(name, n, v);
+    }
+private:
+    template <typename T>
+    std::vector<float> natural(const T *data, int cells,
+                               Ordering3D ordering) const
+    {
+        if (data == nullptr) { throw std::invalid_argument("data is NULL"); }
+        const int nx = mGeometry.getNumberOfGridPointsInX() - cells;
+        const int ny = mGeometry.getNumberOfGridPointsInY() - cells;
+        const int nz = mGeometry.getNumberOfGridPointsInZ() - cells;
+        const size_t n = (size_t) nx*ny*nz;
+        if (ordering == Ordering3D::Natural)
+        {
+            return std::vector<float>(data, data + n);
+        }
+        std::vector<float> v(n);
+        for (int iz = 0; iz < nz; ++iz)
+        {
+            for (int iy = 0; iy < ny; ++iy)
+            {
+                for (int ix = 0; ix < nx; ++ix)
+                {
+                    v[detail::natural_index(nx, ny, ix, iy, iz)] =
+                        static_cast<float>(
+                            data[detail::natural_index(nz, ny, iz, iy, ix)]);
+                }
+            }
+        }
+        return v;
+    }
+    Geometry3D mGeometry;
+};
+
+class VTKRectilinearGrid2D : public detail::RectilinearWriter
+{
+public:
+    ~VTKRectilinearGrid2D() { close(); }
+    void open(const std::string &fileName, const Geometry2D &geometry,
+              const std::string &title = "traveltimes",
+              bool writeBinary = true)
+    {
+        (void) geometry.c();
+        mGeometry = geometry;
+        const int n[3] = {geometry.getNumberOfGridPointsInX(), 1,
+                          geometry.getNumberOfGridPointsInZ()};
+        const float dx = (float) geometry.getGridSpacingInX();
+        const float dz = (float) geometry.getGridSpacingInZ();
+        const float x0 = (float) geometry.getOriginInX();
+        const float z0 = (float) geometry.getOriginInZ();
+        std::vector<float> c[3], a[3];
+        for (int i = 0; i < n[0]; ++i)
+        {
+            c[0].push_back(x0 + dx*i);
+            a[0].push_back(x0 + dx*i);
+        }
+        c[1].push_back(0.f);
+        a[1].push_back(0.f);
+        for (int i = 0; i < n[2]; ++i)
+        {
+            c[2].push_back(-(z0 + dz*i));
+            a[2].push_back(z0 - dz*i);
+        }
+        start(fileName, title, writeBinary, n, c, a);
+    }
+    template <typename T>
+    void writeNodalDataset(const std::string &name, const T *data,
+                           Ordering2D ordering = Ordering2D::Natural) const
+    {
+        scalars(true, name, natural(data, 0, ordering));
+    }
+    template <typename T>
+    void writeCellularDataset(const std::string &name, const T *data,
+                              Ordering2D ordering = Ordering2D::Natural) const
+    {
+        scalars(false, name, natural(data, 1, ordering));
+    }
+    // (vx, vz) per node, written as (vx, 0, vz).
+    template <typename T>
+    void writeNodalVectorDataset(const std::string &name, const T *data,
+                                 Ordering2D ordering = Ordering2D::Natural)
+        const
+    {
+        if (data == nullptr) { throw std::invalid_argument("data is NULL"); }
+        const int nx = mGeometry.getNumberOfGridPointsInX();
+        const int nz = mGeometry.getNumberOfGridPointsInZ();
+        std::vector<float> v(3*(size_t) nx*nz);
+        for (int iz = 0; iz < nz; ++iz)
+        {
+            for (int ix = 0; ix < nx; ++ix)
+            {
+                const size_t dst = (size_t) iz*nx + ix;
+                const size_t src = ordering == Ordering2D::Natural
+                                       ? dst : (size_t) ix*nz + iz;
+                v[3*dst] = static_cast<float>(data[2*src]);
+                v[3*dst + 1] = 0.f;
+                v[3*dst + 2] = static_cast<float>(data[2*src + 1]);
+            }
+        }
+        vectors(name, (size_t) nx*nz, v);
+    }
+private:
+    template <typename T>
+    std::vector<float> natural(const T *data, int cells,
+                               Ordering2D ordering) const
+    {
+        if (data == nullptr) { throw std::invalid_argument("data is NULL"); }
+        const int nx = mGeometry.getNumberOfGridPointsInX() - cells;
+        const int nz = mGeometry.getNumberOfGridPointsInZ() - cells;
+        const size_t n = (size_t) nx*nz;
+        if (ordering == Ordering2D::Natural)
+        {
+            return std::vector<float>(data, data + n);
+        }
+        std::vector<float> v(n);
+        for (int iz = 0; iz < nz; ++iz)
+        {
+            for (int ix = 0; ix < nx; ++ix)
+            {
+                v[(size_t) iz*nx + ix] =
+                    static_cast<float>(data[(size_t) ix*nz + iz]);
+            }
+        }
+        return v;
+    }
+    Geometry2D mGeometry;
+};
+} // namespace IO
+
+// ---------------------------------------------------------------------------
 // Model3D<T> / Model2D<T> (include/eikonalxx/model3d.hpp:18-150).  The
 // model keeps the caller's velocities; the cell slowness is formed on the
 // GPU when a solver takes the model (src/model3d.cpp:45-151).
@@ -571,6 +938,40 @@ public:
             }
         }
         return static_cast<T>(8.0/sum);
+    }
+    // Cell velocities, natural order: the reciprocal of each cell's
+    // slowness (src/model3d.cpp:413-432).
+    [[nodiscard]] std::vector<T> getVelocities() const
+    {
+        detail::require(mHave, "Velocity model not yet set");
+        const int ncx = mGeometry.getNumberOfCellsInX();
+        const int ncy = mGeometry.getNumberOfCellsInY();
+        const int ncz = mGeometry.getNumberOfCellsInZ();
+        std::vector<T> v((size_t) ncx*ncy*ncz);
+        const T one = 1;
+        size_t i = 0;
+        for (int iz = 0; iz < ncz; ++iz)
+        {
+            for (int iy = 0; iy < ncy; ++iy)
+            {
+                for (int ix = 0; ix < ncx; ++ix)
+                {
+                    v[i++] = one/getSlowness(ix, iy, iz);
+                }
+            }
+        }
+        return v;
+    }
+    // src/model3d.cpp:469-480: the cell velocities as a binary cellular
+    // dataset.
+    void writeVTK(const std::string &fileName,
+                  const std::string &title = "velocity_m/s") const
+    {
+        const std::vector<T> v = getVelocities();
+        IO::VTKRectilinearGrid3D writer;
+        writer.open(fileName, mGeometry, title, true);
+        writer.writeCellularDataset(title, v.data(), Ordering3D::Natural);
+        writer.close();
     }
     void clear() noexcept
     {
@@ -672,6 +1073,54 @@ public:
         mHave = true;
     }
     [[nodiscard]] bool haveVelocities() const noexcept { return mHave; }
+    // Slowness of one cell: 4 over the sum of its corner velocities
+    // (src/model2d.cpp:46-127).
+    [[nodiscard]] T getSlowness(int iCellX, int iCellZ) const
+    {
+        detail::require(mHave, "Velocity model not yet set");
+        const int ncx = mGeometry.getNumberOfCellsInX();
+        const int ncz = mGeometry.getNumberOfCellsInZ();
+        if (iCellX < 0 || iCellX >= ncx || iCellZ < 0 || iCellZ >= ncz)
+        {
+            throw std::invalid_argument("cell index out of range");
+        }
+        const int nx = ncx + 1, nz = ncz + 1;
+        auto at = [&](int x, int z) {
+            return mOrdering == Ordering2D::Natural
+                       ? mVelocity[(size_t) z*nx + x]
+                       : mVelocity[(size_t) x*nz + z];
+        };
+        const double sum1 = at(iCellX, iCellZ) + at(iCellX + 1, iCellZ);
+        const double sum2 = at(iCellX, iCellZ + 1) +
+                            at(iCellX + 1, iCellZ + 1);
+        return static_cast<T>(4.0/(sum1 + sum2));
+    }
+    [[nodiscard]] std::vector<T> getVelocities() const
+    {
+        detail::require(mHave, "Velocity model not yet set");
+        const int ncx = mGeometry.getNumberOfCellsInX();
+        const int ncz = mGeometry.getNumberOfCellsInZ();
+        std::vector<T> v((size_t) ncx*ncz);
+        const T one = 1;
+        for (int iz = 0; iz < ncz; ++iz)
+        {
+            for (int ix = 0; ix < ncx; ++ix)
+            {
+                v[(size_t) iz*ncx + ix] = one/getSlowness(ix, iz);
+            }
+        }
+        return v;
+    }
+    // src/model2d.cpp:434-445.
+    void writeVTK(const std::string &fileName,
+                  const std::string &title = "velocity_m/s") const
+    {
+        const std::vector<T> v = getVelocities();
+        IO::VTKRectilinearGrid2D writer;
+        writer.open(fileName, mGeometry, title, true);
+        writer.writeCellularDataset(title, v.data(), Ordering2D::Natural);
+        writer.close();
+    }
     void clear() noexcept
     {
         mGeometry.clear();
@@ -786,13 +1235,28 @@ public:
     {
         return ekx_solver3d_have_source(mHandle) != 0;
     }
-    // Stored only, as src/solver3d.cpp:555-609 does.
+    // src/solver3d.cpp:555-609.  Every solve also interpolates its field
+    // at the stations (getTravelTimesAtStations).
     void setStations(const std::vector<Station3D> &stations)
     {
         detail::require(isInitialized(), "Class not initialized");
+        std::vector<double> x, y, z;
+        for (const auto &s : stations)
+        {
+            x.push_back(s.getLocationInX());
+            y.push_back(s.getLocationInY());
+            z.push_back(s.getLocationInZ());
+        }
+        detail::check(ekx_solver3d_set_stations(mHandle, (int) x.size(),
+                                                x.data(), y.data(),
+                                                z.data()));
         mStations = stations;
     }
     [[nodiscard]] std::vector<Station3D> getStations() const
+    {
+        return mStations;
+    }
+    [[nodiscard]] const std::vector<Station3D> &getStationsReference() const
     {
         return mStations;
     }
@@ -800,6 +1264,63 @@ public:
     {
         detail::check(ekx_solver3d_solve(mHandle));
         mField.clear();
+        mGradient.clear();
+    }
+    // Trilinear travel times at the stations (the 3D form of
+    // src/solver2d.cpp:106-167).
+    [[nodiscard]] std::vector<T> getTravelTimesAtStations() const
+    {
+        detail::require(haveTravelTimeField(),
+                        "Travel time field not yet computed");
+        std::vector<double> out(mStations.size());
+        detail::check(ekx_solver3d_get_travel_times_at_stations(
+            mHandle, (int) out.size(), out.data()));
+        return std::vector<T>(out.begin(), out.end());
+    }
+    void computeTravelTimeGradientField()
+    {
+        detail::check(
+            ekx_solver3d_compute_travel_time_gradient_field(mHandle));
+        mGradient.clear();
+    }
+    [[nodiscard]] bool haveTravelTimeGradientField() const noexcept
+    {
+        return ekx_solver3d_have_travel_time_gradient_field(mHandle) != 0;
+    }
+    // (dT/dx, dT/dy, dT/dz) per node, natural order (solver3d.hpp:153).
+    [[nodiscard]] std::vector<T> getTravelTimeGradientField() const
+    {
+        detail::require(haveTravelTimeGradientField(),
+                        "Travel time gradient field not yet computed");
+        const int64_t n = 3*(int64_t) mGeometry.getNumberOfGridPoints();
+        std::vector<double> out((size_t) n);
+        detail::check(ekx_solver3d_get_travel_time_gradient_field(
+            mHandle, n, out.data()));
+        return std::vector<T>(out.begin(), out.end());
+    }
+    [[nodiscard]] const T *getTravelTimeGradientFieldPointer() const
+    {
+        if (mGradient.empty()) { mGradient = getTravelTimeGradientField(); }
+        return mGradient.data();
+    }
+    // The field (and optionally its gradient, as 3-vectors) as a binary
+    // VTK rectilinear grid -- Solver2D::writeVTK (src/solver2d.cpp:946-964)
+    // on three axes.
+    void writeVTK(const std::string &fileName,
+                  const std::string &title = "travel_time_field_s",
+                  bool writeGradient = false) const
+    {
+        const T *t = getTravelTimeFieldPointer();
+        const T *g = writeGradient ? getTravelTimeGradientFieldPointer()
+                                   : nullptr;
+        IO::VTKRectilinearGrid3D writer;
+        writer.open(fileName, mGeometry, title, true);
+        writer.writeNodalDataset(title, t, Ordering3D::Natural);
+        if (writeGradient)
+        {
+            writer.writeNodalVectorDataset(title + "_gradient", g);
+        }
+        writer.close();
     }
     [[nodiscard]] bool haveTravelTimeField() const noexcept
     {
@@ -842,6 +1363,7 @@ public:
         mHandle = nullptr;
         (void) ekx_solver3d_create(&mHandle);
         mField.clear();
+        mGradient.clear();
         mStations.clear();
         mModel.clear();
     }
@@ -853,6 +1375,7 @@ private:
     Source3D mSource;
     std::vector<Station3D> mStations;
     mutable std::vector<T> mField;
+    mutable std::vector<T> mGradient;
 };
 
 // ---------------------------------------------------------------------------
@@ -955,6 +1478,52 @@ public:
         detail::check(ekx_solver2d_solve(mHandle));
         mHaveField = true;
         mField.clear();
+        mGradient.clear();
+    }
+    // src/solver2d.cpp:866-879
+    void computeTravelTimeGradientField()
+    {
+        detail::check(
+            ekx_solver2d_compute_travel_time_gradient_field(mHandle));
+        mGradient.clear();
+    }
+    [[nodiscard]] bool haveTravelTimeGradientField() const noexcept
+    {
+        return ekx_solver2d_have_travel_time_gradient_field(mHandle) != 0;
+    }
+    // (dT/dx, dT/dz) per node (src/solver2d.cpp:925-932).
+    [[nodiscard]] std::vector<T> getTravelTimeGradientField() const
+    {
+        detail::require(haveTravelTimeGradientField(),
+                        "Travel time gradient field not yet computed");
+        const int64_t n = 2*(int64_t) mGeometry.getNumberOfGridPoints();
+        std::vector<double> out((size_t) n);
+        detail::check(ekx_solver2d_get_travel_time_gradient_field(
+            mHandle, n, out.data()));
+        return std::vector<T>(out.begin(), out.end());
+    }
+    [[nodiscard]] const T *getTravelTimeGradientFieldPointer() const
+    {
+        if (mGradient.empty()) { mGradient = getTravelTimeGradientField(); }
+        return mGradient.data();
+    }
+    // src/solver2d.cpp:946-964
+    void writeVTK(const std::string &fileName,
+                  const std::string &title = "travel_time_field_s",
+                  bool writeGradient = false) const
+    {
+        const T *t = getTravelTimeFieldPointer();
+        const T *g = writeGradient ? getTravelTimeGradientFieldPointer()
+                                   : nullptr;
+        IO::VTKRectilinearGrid2D writer;
+        writer.open(fileName, mGeometry, title, true);
+        writer.writeNodalDataset(title, t, Ordering2D::Natural);
+        if (writeGradient)
+        {
+            writer.writeNodalVectorDataset(title + "_gradient", g,
+                                           Ordering2D::Natural);
+        }
+        writer.close();
     }
     [[nodiscard]] bool haveTravelTimeField() const noexcept
     {
@@ -1000,6 +1569,7 @@ private:
     bool mInitialized = false, mHaveModel = false, mHaveSource = false;
     bool mHaveField = false;
     mutable std::vector<T> mField;
+    mutable std::vector<T> mGradient;
 };
 
 } // namespace eikonalxx_amd
