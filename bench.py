@@ -122,6 +122,7 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                              chains=args.chains,
                              prefetch_planes=args.prefetch,
                              chunk_planes=args.chunk,
+                             column_group=args.column_group,
                              use_graph=not args.no_graph)
     batch = ekx.Batch3D()
     batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
@@ -175,7 +176,8 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                                   precision=getattr(ekx.Precision, mode),
                                   kernel_timing=True, chains=args.chains,
                                   prefetch_planes=args.prefetch,
-                                  chunk_planes=args.chunk)
+                                  chunk_planes=args.chunk,
+                                  column_group=args.column_group)
         tb = ekx.Batch3D()
         tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
         tb.solve()
@@ -269,6 +271,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=0,
                     help="z-planes per column chunk (0: the library's "
                          "choice)")
+    ap.add_argument("--column-group", type=int, default=0,
+                    help="1: grouped refinement launches for batches on "
+                         "one model (0: one problem per wave)")
     ap.add_argument("--chains", type=int, default=0,
                     help="independent launch chains (0: the library's "
                          "choice)")

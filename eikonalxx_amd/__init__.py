@@ -79,7 +79,8 @@ class _Options(ctypes.Structure):
                 ("kernel_timing", ctypes.c_int),
                 ("chains", ctypes.c_int),
                 ("prefetch_planes", ctypes.c_int),
-                ("chunk_planes", ctypes.c_int)]
+                ("chunk_planes", ctypes.c_int),
+                ("column_group", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -105,7 +106,7 @@ class SolveStats(ctypes.Structure):
                 ("timed_launches", ctypes.c_int64),
                 ("timed_updates", ctypes.c_int64),
                 ("iterations_run", ctypes.c_int32),
-                ("reserved", ctypes.c_int32),
+                ("group_waves", ctypes.c_int32),
                 ("timed_kernel_ms", ctypes.c_double),
                 ("last_iteration_change", ctypes.c_double),
                 ("last_solve_ms", ctypes.c_double),
@@ -113,8 +114,7 @@ class SolveStats(ctypes.Structure):
                 ("sweep_ms_fp64", ctypes.c_double)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_
-                if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class _Problem3D(ctypes.Structure):
@@ -294,7 +294,8 @@ class SolverOptions:
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
                  use_graph=True, verbosity=0, kernel_timing=False,
-                 chains=0, prefetch_planes=0, chunk_planes=0):
+                 chains=0, prefetch_planes=0, chunk_planes=0,
+                 column_group=0):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -314,6 +315,7 @@ class SolverOptions:
         o.chains = int(chains)
         o.prefetch_planes = int(prefetch_planes)
         o.chunk_planes = int(chunk_planes)
+        o.column_group = int(column_group)
         o.verbosity = int(verbosity)
         self._o = o
 

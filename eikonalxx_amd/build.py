@@ -4,7 +4,11 @@ Usage: python -m eikonalxx_amd.build [--verbose]
 
 Every translation unit is compiled with -ffp-contract=off (the double
 precision sweeps keep the reference's operation order, so they reproduce it
-bit for bit) and with packed FP32 instructions disabled.
+bit for bit), with -fno-honor-nans (no value of the solver is ever NaN: times
+are finite or the 1e10 sentinel and square roots are taken of positive
+arguments only, so fmin/fmax need no NaN-quieting of their operands -- one
+canonicalising instruction per operand in IEEE mode) and with packed FP32
+instructions disabled.
 """
 import concurrent.futures
 import os
@@ -30,7 +34,7 @@ def hipcc():
 
 def compile_flags():
     return ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
-            "-ffp-contract=off", "-fvisibility=hidden",
+            "-ffp-contract=off", "-fno-honor-nans", "-fvisibility=hidden",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-function",
             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -162,6 +162,10 @@ ekx_status check_options(const ekx_solver_options *o)
     {
         return fail(EKX_EINVAL, "chunk_planes must be >= 0");
     }
+    if (o->column_group != 0 && o->column_group != 1)
+    {
+        return fail(EKX_EINVAL, "column_group must be 0 or 1");
+    }
     if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
         o->prefetch_planes != 8 && o->prefetch_planes != 12)
     {
@@ -353,6 +357,18 @@ struct Engine3D
     std::vector<int> chainBegin;
     std::vector<hipStream_t> chainStreams;
     std::vector<hipEvent_t> chainEvents;
+    // Grouped refinement sweeps (every problem on one model and one
+    // geometry): per group size G (column_group<TS>() of the element types
+    // in use), items {first problem of a group of G, bx, by, bz} ordered by
+    // chain, then diagonal, as diagStart orders the single items.
+    struct GroupItems
+    {
+        int G = 0;
+        DeviceBuffer<ColumnItem> items;
+        std::vector<int> diagStart;
+    };
+    GroupItems groupItems[2];
+    bool grouped = false;
     // Upper bound, over any source position, of the columns per diagonal
     // that overlap one problem's factored box: the box spans at most
     // ceil((2r+1)/8)+1 columns per axis.
@@ -676,8 +692,105 @@ struct Engine3D
         EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
                                flat.size()*sizeof(ColumnItem),
                                hipMemcpyHostToDevice, stream));
+        EKX_CHECK(setup_groups(geoms, nModels));
         EKX_HIP(hipStreamSynchronize(stream));
         return EKX_OK;
+    }
+
+    // Group work lists, when every problem shares the one model and the
+    // geometry and every chain holds at least one full group.
+    ekx_status setup_groups(const std::vector<ekx_geometry3d> &geoms,
+                            int nModels)
+    {
+        grouped = false;
+        for (GroupItems &g : groupItems)
+        {
+            g.G = 0;
+            g.items.release();
+            g.diagStart.clear();
+        }
+        const int n = (int) geoms.size();
+        if (opt.column_group != 1 || kernel() != EKX_KERNEL_COLUMN ||
+            nModels != 1)
+        {
+            return EKX_OK;
+        }
+        const ekx_geometry3d &g0 = geoms[0];
+        for (const ekx_geometry3d &g : geoms)
+        {
+            if (g.nx != g0.nx || g.ny != g0.ny || g.nz != g0.nz ||
+                g.dx != g0.dx || g.dy != g0.dy || g.dz != g0.dz)
+            {
+                return EKX_OK;
+            }
+        }
+        // group sizes of the element types the solve sweeps with
+        std::vector<int> sizes;
+        if (sweeps_f64()) { sizes.push_back(column_group<double>()); }
+        else if (fixed32()) { sizes.push_back(column_group<uint32_t>()); }
+        else { sizes.push_back(column_group<float>()); }
+        if (polish() && sizes[0] != column_group<double>())
+        {
+            sizes.push_back(column_group<double>());
+        }
+        for (int c = 0; c < nChains; ++c)
+        {
+            for (int G : sizes)
+            {
+                if (chainBegin[c + 1] - chainBegin[c] < G) { return EKX_OK; }
+            }
+        }
+        const int nbx = (g0.nx + kColumnTile - 1)/kColumnTile;
+        const int nby = (g0.ny + kColumnTile - 1)/kColumnTile;
+        const int nbz = (g0.nz + chunkH - 1)/chunkH;
+        for (size_t k = 0; k < sizes.size(); ++k)
+        {
+            GroupItems &gi = groupItems[k];
+            gi.G = sizes[k];
+            std::vector<ColumnItem> flat;
+            for (int c = 0; c < nChains; ++c)
+            {
+                std::vector<std::vector<ColumnItem>> perDiag(nDiag);
+                for (int first = chainBegin[c]; first < chainBegin[c + 1];
+                     first += gi.G)
+                {
+                    for (int bz = 0; bz < nbz; ++bz)
+                    {
+                        for (int by = 0; by < nby; ++by)
+                        {
+                            for (int bx = 0; bx < nbx; ++bx)
+                            {
+                                perDiag[bx + by + bz].push_back(
+                                    ColumnItem{first, bx, by, bz});
+                            }
+                        }
+                    }
+                }
+                for (auto &d : perDiag)
+                {
+                    gi.diagStart.push_back((int) flat.size());
+                    flat.insert(flat.end(), d.begin(), d.end());
+                }
+                gi.diagStart.push_back((int) flat.size());
+            }
+            EKX_CHECK(gi.items.allocate(flat.size()));
+            EKX_HIP(hipMemcpyAsync(gi.items.ptr, flat.data(),
+                                   flat.size()*sizeof(ColumnItem),
+                                   hipMemcpyHostToDevice, stream));
+        }
+        grouped = true;
+        return EKX_OK;
+    }
+
+    // The group work list for element type TS, or NULL.
+    template <typename TS> const GroupItems *group_items() const
+    {
+        if (!grouped) { return nullptr; }
+        for (const GroupItems &g : groupItems)
+        {
+            if (g.G == column_group<TS>()) { return &g; }
+        }
+        return nullptr;
     }
 
     // Uploads one model (cell slowness, double) to its slot.
@@ -791,6 +904,14 @@ struct Engine3D
         p.sStep = sMax/(double) kTimeMaxCode;
     }
 
+    // kernel_timing times the launches of the dominant bulk kernel: the
+    // grouped refinement launches when the solve groups, else every bulk
+    // column launch.
+    template <typename TS> bool timed_sweep(bool initialize) const
+    {
+        return timing && (initialize ? group_items<TS>() == nullptr : true);
+    }
+
     // Work counters of one sweep over every problem.
     template <typename TS>
     void count_sweep(int sweep, bool initialize, ekx_solve_stats &st) const
@@ -804,8 +925,13 @@ struct Engine3D
             if (sizeof(TS) == 8) { st.updates_fp64 += upd; }
             if (kernel() == EKX_KERNEL_COLUMN)
             {
-                st.updates_factored += host_box_updates(p, sweep,
-                                                        initialize, chunkH);
+                const int64_t box = host_box_updates(p, sweep, initialize,
+                                                     chunkH);
+                st.updates_factored += box;
+                if (timed_sweep<TS>(initialize))
+                {
+                    st.timed_updates += upd - box;
+                }
             }
         }
     }
@@ -833,18 +959,36 @@ struct Engine3D
         hipStream_t cs = chainStreams[c];
         const int p0 = chainBegin[c];
         const int np = chainBegin[c + 1] - p0;
-        const int *ds = diagStart.data() + (size_t) c*(nDiag + 1);
+        const GroupItems *gi = initialize ? nullptr : group_items<TS>();
+        const int *ds = (gi ? gi->diagStart.data() : diagStart.data()) +
+                        (size_t) c*(nDiag + 1);
         for (int d = 0; d < nDiag; ++d)
         {
             const int count = ds[d + 1] - ds[d];
             if (count > 0)
             {
-                if (timing) { EKX_CHECK(record_timing_event(cs)); }
-                EKX_HIP(launch_columns3d<TS>(dProblems.ptr, dItems.ptr + ds[d],
-                                             count, sweep, initialize ? 1 : 0,
-                                             d, boxPerDiag, chunkH, T, S,
-                                             change.ptr, depth(), cs));
-                if (timing) { EKX_CHECK(record_timing_event(cs)); }
+                if (timed_sweep<TS>(initialize))
+                {
+                    EKX_CHECK(record_timing_event(cs));
+                }
+                if (gi)
+                {
+                    EKX_HIP(launch_group_columns3d<TS>(
+                        dProblems.ptr, gi->items.ptr + ds[d], count, sweep,
+                        d, boxPerDiag, chunkH, p0 + np, T, S, change.ptr,
+                        cs));
+                }
+                else
+                {
+                    EKX_HIP(launch_columns3d<TS>(
+                        dProblems.ptr, dItems.ptr + ds[d], count, sweep,
+                        initialize ? 1 : 0, d, boxPerDiag, chunkH, T, S,
+                        change.ptr, depth(), cs));
+                }
+                if (timed_sweep<TS>(initialize))
+                {
+                    EKX_CHECK(record_timing_event(cs));
+                }
                 launches += 1;
             }
             if (boxPerDiag > 0)
@@ -888,9 +1032,11 @@ struct Engine3D
             // Per diagonal: the factored-box columns run on the side stream
             // concurrently with the bulk (Cartesian) launch; both join
             // before the next diagonal.
+            const GroupItems *gi = initialize ? nullptr : group_items<TS>();
+            const std::vector<int> &ds = gi ? gi->diagStart : diagStart;
             for (int d = 0; d < nDiag; ++d)
             {
-                int count = diagStart[d + 1] - diagStart[d];
+                int count = ds[d + 1] - ds[d];
                 if (boxPerDiag > 0)
                 {
                     EKX_HIP(hipEventRecord(evFork, stream));
@@ -900,13 +1046,27 @@ struct Engine3D
                         boxPerDiag, chunkH, T, S, change.ptr, sideStream));
                     launches += 1;
                 }
-                if (timing) { EKX_CHECK(record_timing_event(stream)); }
-                EKX_HIP(launch_columns3d<TS>(dProblems.ptr,
-                                             dItems.ptr + diagStart[d], count,
-                                             sweep, initialize ? 1 : 0, d,
-                                             boxPerDiag, chunkH, T, S,
-                                             change.ptr, depth(), stream));
-                if (timing) { EKX_CHECK(record_timing_event(stream)); }
+                if (timed_sweep<TS>(initialize))
+                {
+                    EKX_CHECK(record_timing_event(stream));
+                }
+                if (gi)
+                {
+                    EKX_HIP(launch_group_columns3d<TS>(
+                        dProblems.ptr, gi->items.ptr + ds[d], count, sweep,
+                        d, boxPerDiag, chunkH, n, T, S, change.ptr, stream));
+                }
+                else
+                {
+                    EKX_HIP(launch_columns3d<TS>(
+                        dProblems.ptr, dItems.ptr + ds[d], count, sweep,
+                        initialize ? 1 : 0, d, boxPerDiag, chunkH, T, S,
+                        change.ptr, depth(), stream));
+                }
+                if (timed_sweep<TS>(initialize))
+                {
+                    EKX_CHECK(record_timing_event(stream));
+                }
                 launches += 1;
                 if (boxPerDiag > 0)
                 {
@@ -956,6 +1116,11 @@ struct Engine3D
         for (int k = 0; k < iterations; ++k)
         {
             for (int s = 0; s < 8; ++s) { list.push_back({s, false}); }
+        }
+        if (iterations > 0 && kernel() == EKX_KERNEL_COLUMN &&
+            group_items<TS>() != nullptr)
+        {
+            st.group_waves = column_group<TS>();
         }
         if (nChains == 1)
         {
@@ -1043,6 +1208,8 @@ struct Engine3D
         st.kernel_launches += local.kernel_launches;
         st.kernel_launches_fp64 += local.kernel_launches_fp64;
         st.iterations_run += local.iterations_run;
+        st.timed_updates += local.timed_updates;
+        st.group_waves = std::max(st.group_waves, local.group_waves);
         return EKX_OK;
     }
 
@@ -1130,9 +1297,6 @@ struct Engine3D
             EKX_CHECK(run_primary<float>(T32.ptr, S32.ptr, itPrimary, st));
             EKX_HIP(hipEventRecord(evSeg[1], stream));
         }
-        // primary-segment work, for the timed-launch accounting
-        const int64_t primaryUpdates = st.updates;
-        const int64_t primaryFactored = st.updates_factored;
         if (polish())
         {
             if (fixed32())
@@ -1175,7 +1339,7 @@ struct Engine3D
             }
             st.timed_kernel_ms = total;
             st.timed_launches = (int64_t) (timingUsed/2);
-            st.timed_updates = primaryUpdates - primaryFactored;
+            // timed_updates: counted per timed sweep by count_sweep
         }
         stats = st;
         haveField = true;
@@ -1414,6 +1578,7 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->chains = 0;
     o->prefetch_planes = 0;
     o->chunk_planes = 0;
+    o->column_group = 0;
 }
 
 ekx_status ekx_device_count(int *count)

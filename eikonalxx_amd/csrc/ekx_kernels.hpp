@@ -14,6 +14,16 @@ template <typename TS> constexpr int column_waves()
     return sizeof(TS) == 4 ? 2 : 1;
 }
 
+// Waves (problems) per block of the grouped column kernel: one slowness
+// ring per block serves them all.  FP64: 6 travel-time rings of 11.4 KB and
+// the shared ring, 80 KB, two blocks (12 waves) per CU, within the 3 waves
+// per SIMD that its register count allows; 4-byte types: 8 rings of
+// 5.7 KB, three blocks per CU.
+template <typename TS> constexpr int column_group()
+{
+    return sizeof(TS) == 4 ? 8 : 6;
+}
+
 template <typename TS>
 hipError_t launch_init3d(const Problem3D *dProblems, int nProblems,
                          int64_t maxNodes, TS *T, hipStream_t stream);
@@ -33,6 +43,17 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
                             int chunk, TS *T, const TS *S,
                             unsigned int *change, int depth,
                             hipStream_t stream);
+// Refinement sweeps (initialize = 0) of problems that share one model and
+// one geometry: items list the first problem of each group of
+// column_group<TS>() consecutive problems; problems from problemEnd on
+// are padding (no updates, no stores).
+template <typename TS>
+hipError_t launch_group_columns3d(const Problem3D *dProblems,
+                                  const ColumnItem *dItems, int nItems,
+                                  int sweep, int diagonal, int boxPerDiag,
+                                  int chunk, int problemEnd, TS *T,
+                                  const TS *S, unsigned int *change,
+                                  hipStream_t stream);
 // The same diagonal's chunks that overlap a factored box: up to
 // boxPerDiag per problem (the box's columns in x and y), full local
 // update.  Independent of the launch above (disjoint chunks), so the two

@@ -216,24 +216,40 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
 // derived from the problem's source instead of the item list.  Keeping the
 // factored arithmetic out of the bulk kernel keeps its register count, and
 // so its occupancy, at that of the Cartesian update.
+//
+// GROUP > 1 (refinement sweeps of problems that share one model and one
+// geometry): the block's GROUP waves take the same column of GROUP
+// consecutive problems (item.problem is the first), and one slowness ring
+// serves them all, loaded cooperatively and published by a block barrier
+// every step.  Only the travel-time rings stay per wave, which nearly
+// halves the LDS per wave in FP64 (12 instead of 7 resident waves per CU).
+// Waves past problemEnd (the last group's padding) and waves whose column
+// overlaps their problem's factored box keep stepping (loads, barriers)
+// but update and store nothing.
 // ---------------------------------------------------------------------------
-template <typename TS, int WAVES, int DL, bool FACTORED>
+template <typename TS, int WAVES, int DL, bool FACTORED, int GROUP>
 __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
                 const ColumnItem *__restrict__ items, int nItems, int sweep,
                 int initialize, int diagonal, int boxPerDiag, int chunk,
-                TS *__restrict__ T, const TS *__restrict__ S,
+                int problemEnd, TS *__restrict__ T, const TS *__restrict__ S,
                 unsigned int *__restrict__ change)
 {
+    constexpr bool kGroup = GROUP > 1;
+    static_assert(!kGroup || (WAVES == GROUP && !FACTORED),
+                  "a group is one block of GROUP waves, bulk instance");
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
     // kColumnRing slots plus one spare that absorbs the stores of the
-    // padding steps after the last plane.
-    __shared__ TS ring[WAVES][kColumnRing + 1][2][kSlot];
+    // padding steps after the last plane.  Travel times per wave; slowness
+    // per wave, or one ring for the whole group.
+    __shared__ TS ringT[WAVES][kColumnRing + 1][kSlot];
+    __shared__ TS ringS[kGroup ? 1 : WAVES][kColumnRing + 1][kSlot];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int itemIndex = blockIdx.x*WAVES + wave;
-    if (itemIndex >= nItems) { return; }
+    const int itemIndex = kGroup ? (int) blockIdx.x : blockIdx.x*WAVES + wave;
+    if (itemIndex >= nItems) { return; }      // whole blocks in group mode
     int pIndex, bx, by, bz;
+    bool active = true;
     if (FACTORED)
     {
         // item = (problem, k): the k-th (bx, by) of the box, at the z-chunk
@@ -255,6 +271,12 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         bx = item.bx;
         by = item.by;
         bz = item.bz;
+        if (kGroup)
+        {
+            pIndex += wave;
+            active = pIndex < problemEnd;
+            pIndex = min(pIndex, problemEnd - 1);
+        }
     }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
     const Problem3D &p = problems[pIndex];
@@ -264,7 +286,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         if (bx >= bc.bxLo && bx <= bc.bxHi && by >= bc.byLo &&
             by <= bc.byHi && bz >= bc.bzLo && bz <= bc.bzHi)
         {
-            return;   // handled by the FACTORED instance
+            // handled by the FACTORED instance
+            if (!kGroup) { return; }
+            active = false;
         }
     }
     const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -275,7 +299,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int Y0 = __builtin_amdgcn_readfirstlane(by)*kColumnTile;
     if (X0 + kColumnTile - 1 < o.jx0 || Y0 + kColumnTile - 1 < o.jy0)
     {
-        return;   // column entirely outside the source octant
+        // column entirely outside the source octant (groups run
+        // refinement sweeps only, whose range is the same for the group)
+        return;
     }
     // This item's planes: the chunk's [bz*chunk, (bz+1)*chunk) within the
     // sweep's range, after a read-only first plane (the previous chunk's
@@ -297,36 +323,36 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int64_t nxy = (int64_t) nx*ny;
     const int64_t ncxy = (int64_t) ncx*ncy;
 
-    // LDS plane layout (index of local coordinate (c, r), c = jx - X0 + 1,
-    // r = jy - Y0 + 1, both in 0..8): the 8 x 8 core row-major with stride
-    // 8 at 0..63 (conflict-free for the lanes' own nodes), the halo row
-    // r = 0 at 64..72, the halo column c = 0 at 73..80.  Cells use the same
-    // layout with c, r the cell's local indices.
-    auto lidx = [](int c, int r)
-    {
-        return (r >= 1 && c >= 1) ? (r - 1)*kColumnTile + (c - 1)
-             : (r == 0 ? kColumnTile*kColumnTile + c
-                       : kColumnTile*kColumnTile + kColumnHalo + (r - 1));
-    };
+    // LDS plane layout: local coordinate (c, r), c = jx - X0 + 1,
+    // r = jy - Y0 + 1, both in 0..8, at r*9 + c (row-major 9 x 9, the
+    // upwind halo row and column at r = 0 and c = 0).  A lane's node is
+    // (a+1, b+1): its x-1, y-1 and xy-1 neighbours and its 4 cells of a
+    // plane sit at fixed offsets 9, 1, 0 and 10 from (a, b), so one address
+    // per ring serves all reads.  Cells use the same layout with c, r the
+    // cell's local indices.
+    auto lidx = [](int c, int r) { return r*kColumnHalo + c; };
     // Ring elements this lane loads: e0 = lane, e1 = lane + 64 (< 81).
     // Every prefetch load is unconditional from a clamped, in-bounds
     // address (lanes without a second element reload their first one);
-    // nodes outside the grid are replaced by HUGE when the plane enters the
-    // ring.  Conditional loads would make the compiler wait for them at the
-    // merge point and serialise the prefetch.
+    // conditional loads would make the compiler wait for them at the merge
+    // point and serialise the prefetch.  Halo nodes outside the grid (the
+    // row or column before the first) hold their clamped neighbour's value:
+    // no update reads them, since updated nodes start at index 1 of every
+    // axis and read only index - 1.
     const bool has1 = lane + 64 < kColumnPlane;
-    const int dst0 = lidx(lane%kColumnHalo, lane/kColumnHalo);
-    const int dst1 = has1 ? lidx((lane + 64)%kColumnHalo,
-                                 (lane + 64)/kColumnHalo)
-                          : kColumnPlane;             // padding if unused
+    const int dst0 = lane;
+    const int dst1 = has1 ? lane + 64 : kColumnPlane; // padding if unused
+    // Group mode: thread t of the block loads slowness element t mod 81 of
+    // each plane into the shared ring (repeated elements are written with
+    // the same value), so every wave issues one slowness load per step.
+    const int sElem = kGroup ? (wave*64 + lane)%kColumnPlane : lane;
+    const int dstS = kGroup ? sElem : dst0;
     int tXY0, tXY1, sXY0, sXY1;
-    bool tOk0, tOk1;
     {
-        auto setup = [&](int e, int &tXY, int &sXY, bool &ok)
+        auto setup = [&](int e, int &tXY, int &sXY)
         {
             int c = e%kColumnHalo, r = e/kColumnHalo;
             int gx = X0 - 1 + c, gy = Y0 - 1 + r;
-            ok = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
             int cgx = min(max(gx, 0), nx - 1);
             int cgy = min(max(gy, 0), ny - 1);
             int ax = negX ? nx - 1 - cgx : cgx;
@@ -338,12 +364,16 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             int cy = negY ? ncy - 1 - pcy : pcy;
             sXY = cy*ncx + cx;
         };
-        setup(lane, tXY0, sXY0, tOk0);
-        setup(has1 ? lane + 64 : lane, tXY1, sXY1, tOk1);
+        setup(lane, tXY0, sXY0);
+        setup(has1 ? lane + 64 : lane, tXY1, sXY1);
+        if (kGroup)
+        {
+            int unusedT;
+            setup(sElem, unusedT, sXY0);
+        }
     }
     using St = Store<TS>;
     using R = typename St::R;
-    const TS hugeCode = St::kHugeCode;
     auto loadPlane = [&](int q, TS &vT0, TS &vT1, TS &vS0, TS &vS1)
     {
         const int jz = P0 + q;
@@ -355,7 +385,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         vT0 = Tz[tXY0];
         vS0 = Sz[sXY0];
         vT1 = Tz[tXY1];
-        vS1 = Sz[sXY1];
+        if (!kGroup) { vS1 = Sz[sXY1]; }
     };
 
     TS bT0[DL], bT1[DL], bS0[DL], bS1[DL];
@@ -372,10 +402,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     R pt4 = (R) kHuge, pt5 = (R) kHuge, pt6 = (R) kHuge, pt7 = (R) kHuge;
     R ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
     R maxDrop = 0;                        // largest decrease by this lane
-    // The node is local (a+1, b+1).  Its neighbours (x-1, y-1, xy-1) and
-    // its 4 cells of one plane share these four LDS offsets.
-    const int oA = lidx(a, b), oB = lidx(a + 1, b), oC = lidx(a, b + 1);
-    const int oD = lidx(a + 1, b + 1);
+    // The node is local (a+1, b+1); oA is (a, b), the xy-1 neighbour.
+    constexpr int kB = 1, kC = kColumnHalo, kD = kColumnHalo + 1;
+    const int oA = lidx(a, b);
     // write-back lane: itself, or the nearest in-grid lane of its row/column
     const int wbA = min(a, nx - 1 - X0), wbB = min(b, ny - 1 - Y0);
     const int wbOwn = lidx(wbA + 1, wbB + 1);
@@ -387,7 +416,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // steps only repeat idempotent work: clamped loads, no computation, and
     // a re-store of the last plane's final values).
     const int nSteps = ((nPlanes + kColumnSkew + 1 + DL - 1)/DL)*DL;
-    TS(*myRing)[2][kSlot] = ring[wave];
+    TS(*myT)[kSlot] = ringT[wave];
+    TS(*myS)[kSlot] = ringS[kGroup ? 0 : wave];
     for (int t0 = 0; t0 < nSteps; t0 += DL)
     {
 #pragma unroll
@@ -400,12 +430,12 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             {
                 const int slot = tc < nPlanes ? (tc & (kColumnRing - 1))
                                               : kColumnRing;
-                TS *dT = myRing[slot][0];
-                TS *dS = myRing[slot][1];
-                dT[dst0] = tOk0 ? bT0[u] : hugeCode;
-                dS[dst0] = bS0[u];
-                dT[dst1] = tOk1 ? bT1[u] : hugeCode;
-                dS[dst1] = bS1[u];
+                TS *dT = myT[slot];
+                TS *dS = myS[slot];
+                dT[dst0] = bT0[u];
+                dS[dstS] = bS0[u];
+                dT[dst1] = bT1[u];
+                if (!kGroup) { dS[dst1] = bS1[u]; }
             }
             // (b) prefetch plane tc + DL into the freed registers (past the
             // last plane the load repeats it and is never stored)
@@ -414,23 +444,38 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // (d, read) the plane the last lane finished a step ago; read
             // with the neighbour values below, stored after the update
             const int w = min(max(tc - (kColumnSkew + 1), 0), nPlanes - 1);
-            const TS wbValue = myRing[w & (kColumnRing - 1)][0][wbOwn];
-            // (c) every lane advances its node on plane tc - (a + b)
-            const int q = tc - (a + b);
-            if (valid && q >= 0 && q < nPlanes)
+            const TS wbValue = myT[w & (kColumnRing - 1)][wbOwn];
+            if (kGroup)
             {
-                TS *sT = myRing[q & (kColumnRing - 1)][0];
-                const TS *sS = myRing[q & (kColumnRing - 1)][1];
-                const TS ownCode = sT[oD];
+                // the group's slowness plane entered above is read below by
+                // every wave: publish it (LDS only; global loads in flight
+                // stay in flight)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup",
+                                       "local");
+                __builtin_amdgcn_s_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup",
+                                       "local");
+            }
+            // (c) every lane advances its node on plane tc - (a + b).  The
+            // reads and the register shift run on every lane at every step
+            // (outside the lane's planes they read unused ring slots and
+            // only feed registers that are overwritten before use), so
+            // that the unrolled steps rename the registers instead of
+            // merging them at a branch.
+            const int q = tc - (a + b);
+            {
+                TS *sT = myT[q & (kColumnRing - 1)] + oA;
+                const TS *sS = myS[q & (kColumnRing - 1)] + oA;
+                const TS ownCode = sT[kD];
                 R tOwn = St::time(ownCode, p);
-                const R t1 = St::time(sT[oC], p);
-                const R t2 = St::time(sT[oA], p);
-                const R t3 = St::time(sT[oB], p);
-                const R c00 = St::slowness(sS[oA], p);
-                const R c10 = St::slowness(sS[oB], p);
-                const R c01 = St::slowness(sS[oC], p);
-                const R c11 = St::slowness(sS[oD], p);
-                if (q > 0 && upd)
+                const R t1 = St::time(sT[kC], p);
+                const R t2 = St::time(sT[0], p);
+                const R t3 = St::time(sT[kB], p);
+                const R c00 = St::slowness(sS[0], p);
+                const R c10 = St::slowness(sS[kB], p);
+                const R c01 = St::slowness(sS[kC], p);
+                const R c11 = St::slowness(sS[kD], p);
+                if (upd && active && q > 0 && q < nPlanes)
                 {
                     const int jz = P0 + q;
                     const int iz = negZ ? nz - 1 - jz : jz;
@@ -459,7 +504,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                         const R tNew = St::finite_time(c, p);
                         maxDrop = Math<R>::max(maxDrop, tOwn - tNew);
                         tOwn = tNew;
-                        sT[oD] = c;
+                        sT[kD] = c;
                     }
                 }
                 pt4 = tOwn;
@@ -476,6 +521,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // unchanged nodes store their own value, lanes past the grid
             // edge repeat an edge lane's store), so the prefetch loads above
             // always have the same number of memory operations behind them.
+            if (active)
             {
                 const int jz = P0 + w;
                 const int az = negZ ? nz - 1 - jz : jz;
@@ -691,26 +737,43 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
     switch (depth)
     {
     case 4:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>), grid, block,
-                           0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, T, S,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false, 1>), grid,
+                           block, 0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
                            change);
         break;
     case 8:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false>), grid, block,
-                           0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, T, S,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false, 1>), grid,
+                           block, 0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
                            change);
         break;
     case 12:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false>), grid, block,
-                           0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, T, S,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false, 1>), grid,
+                           block, 0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
                            change);
         break;
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t launch_group_columns3d(const Problem3D *dProblems,
+                                  const ColumnItem *dItems, int nItems,
+                                  int sweep, int diagonal, int boxPerDiag,
+                                  int chunk, int problemEnd, TS *T,
+                                  const TS *S, unsigned int *change,
+                                  hipStream_t stream)
+{
+    if (nItems <= 0) { return hipSuccess; }
+    constexpr int G = column_group<TS>();
+    hipLaunchKernelGGL((sweep_columns3d<TS, G, 4, false, G>),
+                       dim3((unsigned) nItems), dim3(64*G), 0, stream,
+                       dProblems, dItems, nItems, sweep, 0, diagonal,
+                       boxPerDiag, chunk, problemEnd, T, S, change);
     return hipGetLastError();
 }
 
@@ -724,10 +787,10 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
 {
     if (boxPerDiag <= 0) { return hipSuccess; }
     int n = nProblems*boxPerDiag;
-    hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true>), dim3((unsigned) n),
-                       dim3(64), 0, stream, dProblems,
+    hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true, 1>),
+                       dim3((unsigned) n), dim3(64), 0, stream, dProblems,
                        (const ColumnItem *) nullptr, n, sweep, initialize,
-                       diagonal, boxPerDiag, chunk, T, S, change);
+                       diagonal, boxPerDiag, chunk, 0, T, S, change);
     return hipGetLastError();
 }
 
@@ -818,18 +881,27 @@ template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
 template hipError_t launch_columns3d<float>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     float *, const float *, unsigned int *, int, hipStream_t);
+template hipError_t launch_group_columns3d<float>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    float *, const float *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<float>(
     const Problem3D *, int, int, int, int, int, int, float *, const float *,
     unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<double>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     double *, const double *, unsigned int *, int, hipStream_t);
+template hipError_t launch_group_columns3d<double>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    double *, const double *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<double>(
     const Problem3D *, int, int, int, int, int, int, double *, const double *,
     unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<uint32_t>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     uint32_t *, const uint32_t *, unsigned int *, int, hipStream_t);
+template hipError_t launch_group_columns3d<uint32_t>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
+    uint32_t *, const uint32_t *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<uint32_t>(
     const Problem3D *, int, int, int, int, int, int, uint32_t *, const uint32_t *,
     unsigned int *, hipStream_t);

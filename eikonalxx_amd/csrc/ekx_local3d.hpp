@@ -21,6 +21,29 @@
 namespace ekx
 {
 
+// Correctly rounded square root of a double in [2^-767, DBL_MAX]: the
+// reciprocal-square-root estimate refined by one Goldschmidt step and two
+// corrections with exact FMA residuals (d = a - g*g).  This is the sequence
+// the compiler emits for sqrt(double) on gfx950 minus its range handling:
+// below 2^-767 it scales the argument by 2^256 (so that the residuals stay
+// normal) and it returns zero and infinity unchanged, which costs 8 of its
+// 18 instructions.  The callers pass positive determinants of travel-time
+// differences, so the unscaled form serves every lane except for arguments
+// under 2^-767 (zero included), which take the full sqrt.
+__device__ __forceinline__ double sqrt_normal_range(double a)
+{
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a*y;
+    double h = y*0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double d = fma(-g, g, a);
+    h = fma(h, r, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, a);
+    return fma(d, h, g);
+}
+
 template <typename R> struct Math;
 template <> struct Math<double>
 {
@@ -29,7 +52,17 @@ template <> struct Math<double>
     static __device__ __forceinline__ double max(double a, double b)
     { return fmax(a, b); }
     static __device__ __forceinline__ double sqrt(double a)
-    { return ::sqrt(a); }
+    {
+        double r = sqrt_normal_range(a);
+        const bool normal = a >= 0x1p-767;
+        // a wave-uniform branch, so that the full sqrt is not evaluated
+        // alongside (lanes outside the range are overwritten here)
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(!normal) != 0, 0))
+        {
+            r = normal ? r : ::sqrt(a);
+        }
+        return r;
+    }
 };
 template <> struct Math<float>
 {
