@@ -80,7 +80,8 @@ class _Options(ctypes.Structure):
                 ("chains", ctypes.c_int),
                 ("prefetch_planes", ctypes.c_int),
                 ("chunk_planes", ctypes.c_int),
-                ("column_group", ctypes.c_int)]
+                ("column_group", ctypes.c_int),
+                ("sweep_overlap", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -295,7 +296,7 @@ class SolverOptions:
                  convergence_tolerance=None, early_exit=False,
                  use_graph=True, verbosity=0, kernel_timing=False,
                  chains=0, prefetch_planes=0, chunk_planes=0,
-                 column_group=0):
+                 column_group=0, sweep_overlap=0):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -316,6 +317,7 @@ class SolverOptions:
         o.prefetch_planes = int(prefetch_planes)
         o.chunk_planes = int(chunk_planes)
         o.column_group = int(column_group)
+        o.sweep_overlap = int(sweep_overlap)
         o.verbosity = int(verbosity)
         self._o = o
 

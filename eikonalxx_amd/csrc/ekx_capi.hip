@@ -166,6 +166,10 @@ ekx_status check_options(const ekx_solver_options *o)
     {
         return fail(EKX_EINVAL, "column_group must be 0 or 1");
     }
+    if (o->sweep_overlap != 0 && o->sweep_overlap != 1)
+    {
+        return fail(EKX_EINVAL, "sweep_overlap must be 0 or 1");
+    }
     if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
         o->prefetch_planes != 8 && o->prefetch_planes != 12)
     {
@@ -357,6 +361,19 @@ struct Engine3D
     std::vector<int> chainBegin;
     std::vector<hipStream_t> chainStreams;
     std::vector<hipEvent_t> chainEvents;
+    // Sweep overlap: per chain a second stream (odd sweeps of a list), its
+    // end event, and the events recorded after each diagonal of the last
+    // two sweeps ([chain][parity][diagonal]); reqDiag[(c*8 + k)*nDiag + d]
+    // is the last diagonal of sweep k - 1 (mod 8) that must be complete
+    // before chain c's sweep k starts its diagonal d.
+    std::vector<hipStream_t> chainStreams2;
+    std::vector<hipEvent_t> chainEvents2;
+    std::vector<hipEvent_t> diagEvents;
+    std::vector<int> reqDiag;
+    bool overlap() const
+    {
+        return opt.sweep_overlap == 1 && kernel() == EKX_KERNEL_COLUMN;
+    }
     // Grouped refinement sweeps (every problem on one model and one
     // geometry): per group size G (column_group<TS>() of the element types
     // in use), items {first problem of a group of G, bx, by, bz} ordered by
@@ -438,7 +455,10 @@ struct Engine3D
         if (evJoin) { (void) hipEventDestroy(evJoin); }
         for (hipEvent_t &e : timingEvents) { (void) hipEventDestroy(e); }
         for (hipEvent_t &e : chainEvents) { (void) hipEventDestroy(e); }
+        for (hipEvent_t &e : chainEvents2) { (void) hipEventDestroy(e); }
+        for (hipEvent_t &e : diagEvents) { (void) hipEventDestroy(e); }
         for (hipStream_t &c : chainStreams) { (void) hipStreamDestroy(c); }
+        for (hipStream_t &c : chainStreams2) { (void) hipStreamDestroy(c); }
         if (changeHost) { (void) hipHostFree(changeHost); }
         if (sideStream) { (void) hipStreamDestroy(sideStream); }
         for (hipEvent_t &e : evSeg)
@@ -693,7 +713,107 @@ struct Engine3D
                                flat.size()*sizeof(ColumnItem),
                                hipMemcpyHostToDevice, stream));
         EKX_CHECK(setup_groups(geoms, nModels));
+        if (overlap()) { EKX_CHECK(setup_overlap(geoms)); }
         EKX_HIP(hipStreamSynchronize(stream));
+        return EKX_OK;
+    }
+
+    // Dependencies between consecutive sweeps (k - 1, k) for the sweep
+    // overlap.  A column chunk C of sweep k may start once every chunk of
+    // sweep k - 1 that writes a node C reads (its nodes and upwind halo)
+    // or reads a node C writes (the downwind neighbours of its nodes) has
+    // finished: all of them lie in C's node box grown by one node in every
+    // direction, and the last of them in sweep k - 1's order is on the
+    // diagonal (max column x) + (max column y) + (max chunk z) of that box
+    // in sweep k - 1's frame.
+    ekx_status setup_overlap(const std::vector<ekx_geometry3d> &geoms)
+    {
+        reqDiag.assign((size_t) nChains*8*nDiag, -1);
+        // natural index range of permuted [j0, j1] on an axis of n nodes,
+        // grown by one node and clipped
+        auto grown = [](int j0, int j1, int n, int neg, int &lo, int &hi)
+        {
+            j0 = std::max(j0, 0);
+            j1 = std::min(j1, n - 1);
+            lo = neg ? n - 1 - j1 : j0;
+            hi = neg ? n - 1 - j0 : j1;
+            lo = std::max(lo - 1, 0);
+            hi = std::min(hi + 1, n - 1);
+        };
+        // largest tile index of natural [lo, hi] in a frame
+        auto lastTile = [](int lo, int hi, int n, int neg, int tile)
+        {
+            return (neg ? n - 1 - lo : hi)/tile;
+        };
+        for (int c = 0; c < nChains; ++c)
+        {
+            for (int i = chainBegin[c]; i < chainBegin[c + 1]; ++i)
+            {
+                const ekx_geometry3d &g = geoms[i];
+                if (i > chainBegin[c])
+                {
+                    const ekx_geometry3d &h = geoms[i - 1];
+                    if (g.nx == h.nx && g.ny == h.ny && g.nz == h.nz)
+                    {
+                        continue;      // same tiles as the previous problem
+                    }
+                }
+                const int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
+                const int nby = (g.ny + kColumnTile - 1)/kColumnTile;
+                const int nbz = (g.nz + chunkH - 1)/chunkH;
+                for (int k = 0; k < 8; ++k)
+                {
+                    const int kp = (k + 7)%8;
+                    const int nx0 = k & 1, ny0 = (k >> 1) & 1;
+                    const int nz0 = (k >> 2) & 1;
+                    const int nx1 = kp & 1, ny1 = (kp >> 1) & 1;
+                    const int nz1 = (kp >> 2) & 1;
+                    int *req = reqDiag.data() + ((size_t) c*8 + k)*nDiag;
+                    for (int bz = 0; bz < nbz; ++bz)
+                    {
+                        int zl, zh;
+                        grown(bz*chunkH - 1, (bz + 1)*chunkH - 1, g.nz, nz0,
+                              zl, zh);
+                        const int tz = lastTile(zl, zh, g.nz, nz1, chunkH);
+                        for (int by = 0; by < nby; ++by)
+                        {
+                            int yl, yh;
+                            grown(by*kColumnTile, by*kColumnTile + 7, g.ny,
+                                  ny0, yl, yh);
+                            const int ty = lastTile(yl, yh, g.ny, ny1,
+                                                    kColumnTile);
+                            for (int bx = 0; bx < nbx; ++bx)
+                            {
+                                int xl, xh;
+                                grown(bx*kColumnTile, bx*kColumnTile + 7,
+                                      g.nx, nx0, xl, xh);
+                                const int tx = lastTile(xl, xh, g.nx, nx1,
+                                                        kColumnTile);
+                                int &r = req[bx + by + bz];
+                                r = std::max(r, std::min(tx + ty + tz,
+                                                         nDiag - 1));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        while (chainStreams2.size() < chainStreams.size())
+        {
+            hipStream_t s2 = nullptr;
+            hipEvent_t e2 = nullptr;
+            EKX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            EKX_HIP(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
+            chainStreams2.push_back(s2);
+            chainEvents2.push_back(e2);
+        }
+        const size_t nEv = (size_t) nChains*2*nDiag;
+        while (diagEvents.size() < nEv)
+        {
+            hipEvent_t e = nullptr;
+            EKX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            diagEvents.push_back(e);
+        }
         return EKX_OK;
     }
 
@@ -904,6 +1024,13 @@ struct Engine3D
         p.sStep = sMax/(double) kTimeMaxCode;
     }
 
+    // The factored-box columns need their own launch unless the bulk
+    // instance runs them (FP64, one problem per wave).
+    template <typename TS> static bool separate_factored(const void *group)
+    {
+        return !merged_factored<TS>() || group != nullptr;
+    }
+
     // kernel_timing times the launches of the dominant bulk kernel: the
     // grouped refinement launches when the solve groups, else every bulk
     // column launch.
@@ -930,7 +1057,10 @@ struct Engine3D
                 st.updates_factored += box;
                 if (timed_sweep<TS>(initialize))
                 {
-                    st.timed_updates += upd - box;
+                    const bool merged =
+                        !separate_factored<TS>(initialize ? nullptr
+                                                          : group_items<TS>());
+                    st.timed_updates += merged ? upd : upd - box;
                 }
             }
         }
@@ -952,11 +1082,26 @@ struct Engine3D
     // One sweep of chain c on its own stream: per diagonal, the bulk
     // launch over the chain's items, then the factored-box columns of the
     // chain's problems.
+    // With the sweep overlap, odd sweeps of a list run on the chain's
+    // second stream: `parity` selects stream and diagonal events, and
+    // waitPrevious makes each diagonal wait for the previous sweep's
+    // diagonals it depends on.
     template <typename TS>
     ekx_status enqueue_chain_sweep(int c, int sweep, bool initialize, TS *T,
-                                   const TS *S, int64_t &launches)
+                                   const TS *S, int64_t &launches,
+                                   int parity = 0, bool waitPrevious = false)
     {
-        hipStream_t cs = chainStreams[c];
+        hipStream_t cs = parity ? chainStreams2[c] : chainStreams[c];
+        const bool ov = overlap();
+        hipEvent_t *evThis = ov ? diagEvents.data() +
+                                      ((size_t) c*2 + parity)*nDiag
+                                : nullptr;
+        hipEvent_t *evPrev = ov ? diagEvents.data() +
+                                      ((size_t) c*2 + (1 - parity))*nDiag
+                                : nullptr;
+        const int *req = ov ? reqDiag.data() + ((size_t) c*8 + sweep)*nDiag
+                            : nullptr;
+        int waited = -1;
         const int p0 = chainBegin[c];
         const int np = chainBegin[c + 1] - p0;
         const GroupItems *gi = initialize ? nullptr : group_items<TS>();
@@ -965,6 +1110,11 @@ struct Engine3D
         for (int d = 0; d < nDiag; ++d)
         {
             const int count = ds[d + 1] - ds[d];
+            if (waitPrevious && req[d] > waited)
+            {
+                waited = req[d];
+                EKX_HIP(hipStreamWaitEvent(cs, evPrev[waited], 0));
+            }
             if (count > 0)
             {
                 if (timed_sweep<TS>(initialize))
@@ -991,13 +1141,14 @@ struct Engine3D
                 }
                 launches += 1;
             }
-            if (boxPerDiag > 0)
+            if (boxPerDiag > 0 && separate_factored<TS>(gi))
             {
                 EKX_HIP(launch_factored_columns3d<TS>(
                     dProblems.ptr + p0, np, sweep, initialize ? 1 : 0, d,
                     boxPerDiag, chunkH, T, S, change.ptr, cs));
                 launches += 1;
             }
+            if (ov) { EKX_HIP(hipEventRecord(evThis[d], cs)); }
         }
         return EKX_OK;
     }
@@ -1037,7 +1188,8 @@ struct Engine3D
             for (int d = 0; d < nDiag; ++d)
             {
                 int count = ds[d + 1] - ds[d];
-                if (boxPerDiag > 0)
+                const bool side = boxPerDiag > 0 && separate_factored<TS>(gi);
+                if (side)
                 {
                     EKX_HIP(hipEventRecord(evFork, stream));
                     EKX_HIP(hipStreamWaitEvent(sideStream, evFork, 0));
@@ -1068,7 +1220,7 @@ struct Engine3D
                     EKX_CHECK(record_timing_event(stream));
                 }
                 launches += 1;
-                if (boxPerDiag > 0)
+                if (side)
                 {
                     EKX_HIP(hipEventRecord(evJoin, sideStream));
                     EKX_HIP(hipStreamWaitEvent(stream, evJoin, 0));
@@ -1122,7 +1274,7 @@ struct Engine3D
         {
             st.group_waves = column_group<TS>();
         }
-        if (nChains == 1)
+        if (nChains == 1 && !overlap())
         {
             for (const auto &sw : list)
             {
@@ -1132,7 +1284,9 @@ struct Engine3D
         else
         {
             // Chains run their whole sweep list independently: fork from
-            // the main stream, join back at the end.
+            // the main stream, join back at the end.  With the overlap,
+            // sweep i of the list runs on the chain's stream i mod 2.
+            const bool ov = overlap();
             for (const auto &sw : list)
             {
                 count_sweep<TS>(sw.first, sw.second, st);
@@ -1142,16 +1296,31 @@ struct Engine3D
             for (int c = 0; c < nChains; ++c)
             {
                 EKX_HIP(hipStreamWaitEvent(chainStreams[c], evFork, 0));
-                for (const auto &sw : list)
+                if (ov)
                 {
-                    EKX_CHECK(enqueue_chain_sweep<TS>(c, sw.first, sw.second,
-                                                      T, S, launches));
+                    EKX_HIP(hipStreamWaitEvent(chainStreams2[c], evFork, 0));
+                }
+                for (size_t i = 0; i < list.size(); ++i)
+                {
+                    const int parity = ov ? (int) (i & 1) : 0;
+                    EKX_CHECK(enqueue_chain_sweep<TS>(
+                        c, list[i].first, list[i].second, T, S, launches,
+                        parity, ov && i > 0));
                 }
                 EKX_HIP(hipEventRecord(chainEvents[c], chainStreams[c]));
+                if (ov)
+                {
+                    EKX_HIP(hipEventRecord(chainEvents2[c],
+                                           chainStreams2[c]));
+                }
             }
             for (int c = 0; c < nChains; ++c)
             {
                 EKX_HIP(hipStreamWaitEvent(stream, chainEvents[c], 0));
+                if (ov)
+                {
+                    EKX_HIP(hipStreamWaitEvent(stream, chainEvents2[c], 0));
+                }
             }
             st.kernel_launches += launches;
             if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
@@ -1579,6 +1748,7 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->prefetch_planes = 0;
     o->chunk_planes = 0;
     o->column_group = 0;
+    o->sweep_overlap = 0;
 }
 
 ekx_status ekx_device_count(int *count)

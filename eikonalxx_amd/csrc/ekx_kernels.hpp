@@ -14,6 +14,16 @@ template <typename TS> constexpr int column_waves()
     return sizeof(TS) == 4 ? 2 : 1;
 }
 
+// FP64 bulk launches also take the columns that overlap a factored box
+// (full local update in those waves): LDS already holds FP64 at 7 resident
+// waves per CU, below what the larger register count of the factored
+// branch allows, so one launch per diagonal costs no occupancy.  The
+// 4-byte types keep those columns in the separate FACTORED instance.
+template <typename TS> constexpr bool merged_factored()
+{
+    return sizeof(TS) == 8;
+}
+
 // Waves (problems) per block of the grouped column kernel: one slowness
 // ring per block serves them all.  FP64: 6 travel-time rings of 11.4 KB and
 // the shared ring, 80 KB, two blocks (12 waves) per CU, within the 3 waves

@@ -238,6 +238,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     constexpr bool kGroup = GROUP > 1;
     static_assert(!kGroup || (WAVES == GROUP && !FACTORED),
                   "a group is one block of GROUP waves, bulk instance");
+    // bulk instance that also runs the factored-box columns
+    constexpr bool kMerged = !FACTORED && !kGroup && merged_factored<TS>();
+    bool boxColumn = false;
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
     // kColumnRing slots plus one spare that absorbs the stores of the
     // padding steps after the last plane.  Travel times per wave; slowness
@@ -286,9 +289,10 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         if (bx >= bc.bxLo && bx <= bc.bxHi && by >= bc.byLo &&
             by <= bc.byHi && bz >= bc.bzLo && bz <= bc.bzHi)
         {
-            // handled by the FACTORED instance
-            if (!kGroup) { return; }
-            active = false;
+            // the full local update in this wave, or the FACTORED instance
+            if (kMerged) { boxColumn = true; }
+            else if (!kGroup) { return; }
+            else { active = false; }
         }
     }
     const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -480,7 +484,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                     const int jz = P0 + q;
                     const int iz = negZ ? nz - 1 - jz : jz;
                     R tUpd;
-                    if (FACTORED)
+                    if (FACTORED || (kMerged && boxColumn))
                     {
                         tUpd = local_update3d<R>(
                             p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00,

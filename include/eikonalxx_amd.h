@@ -131,6 +131,13 @@ typedef struct
                                       slowness ring (more resident waves,
                                       a block barrier per step); 0
                                       (default): one problem per wave     */
+    int sweep_overlap;             /* column kernel: 1 lets each sweep start
+                                      its first column diagonals while the
+                                      previous sweep finishes its last ones
+                                      (every column waits only for the
+                                      columns of the previous sweep that
+                                      write or read its nodes or halo);
+                                      0 (default): sweep after sweep      */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */
