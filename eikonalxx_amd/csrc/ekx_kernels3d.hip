@@ -378,18 +378,37 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     }
     using St = Store<TS>;
     using R = typename St::R;
-    auto loadPlane = [&](int q, TS &vT0, TS &vT1, TS &vS0, TS &vS1)
+    // Plane addressing: wave-uniform plane pointers (scalar registers)
+    // plus per-lane byte offsets within a plane (32-bit), so the loads and
+    // the store take the scalar-base form and the per-step address work is
+    // a scalar pointer increment.
+    const uint32_t tB0 = (uint32_t) tXY0*sizeof(TS);
+    const uint32_t tB1 = (uint32_t) tXY1*sizeof(TS);
+    const uint32_t sB0 = (uint32_t) sXY0*sizeof(TS);
+    const uint32_t sB1 = (uint32_t) sXY1*sizeof(TS);
+    auto planeT = [&](int q) -> const char *
     {
         const int jz = P0 + q;
         const int az = negZ ? nz - 1 - jz : jz;
-        const int pcz = min(jz, ncz - 1);
+        return (const char *) (Tp + (int64_t) az*nxy);
+    };
+    auto planeS = [&](int q) -> const char *
+    {
+        const int pcz = min(P0 + q, ncz - 1);
         const int cz = negZ ? ncz - 1 - pcz : pcz;
-        const TS *Tz = Tp + (int64_t) az*nxy;
-        const TS *Sz = Sp + (int64_t) cz*ncxy;
-        vT0 = Tz[tXY0];
-        vS0 = Sz[sXY0];
-        vT1 = Tz[tXY1];
-        if (!kGroup) { vS1 = Sz[sXY1]; }
+        return (const char *) (Sp + (int64_t) cz*ncxy);
+    };
+    auto at = [](const char *base, uint32_t off) -> TS
+    {
+        return *(const TS *) (base + off);
+    };
+    auto loadPlane = [&](const char *Tz, const char *Sz, TS &vT0, TS &vT1,
+                         TS &vS0, TS &vS1)
+    {
+        vT0 = at(Tz, tB0);
+        vS0 = at(Sz, sB0);
+        vT1 = at(Tz, tB1);
+        if (!kGroup) { vS1 = at(Sz, sB1); }
     };
 
     TS bT0[DL], bT1[DL], bS0[DL], bS1[DL];
@@ -399,9 +418,21 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
 #pragma unroll
     for (int u = 0; u < DL; ++u)
     {
-        loadPlane(min(u, nPlanes - 1), bT0[u], bT1[u], bS0[u], bS1[u]);
+        const int q = min(u, nPlanes - 1);
+        loadPlane(planeT(q), planeS(q), bT0[u], bT1[u], bS0[u], bS1[u]);
         __builtin_amdgcn_sched_barrier(0);
     }
+    // Running plane pointers: the next prefetch (plane min(tc + DL,
+    // nPlanes - 1)) and the next write-back (plane
+    // clamp(tc - 15, 0, nPlanes - 1)); each advances by at most one plane
+    // per step.
+    const int64_t stepT = (negZ ? -nxy : nxy)*(int64_t) sizeof(TS);
+    const int64_t stepS = (negZ ? -ncxy : ncxy)*(int64_t) sizeof(TS);
+    int loadQ = min(DL, nPlanes - 1);
+    const char *loadT = planeT(loadQ);
+    const char *loadS = planeS(loadQ);
+    int wbQ = 0;
+    char *wbT = (char *) planeT(0);
     // z-neighbours (plane q - 1) of the lane's node, decoded
     R pt4 = (R) kHuge, pt5 = (R) kHuge, pt6 = (R) kHuge, pt7 = (R) kHuge;
     R ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
@@ -414,7 +445,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int wbOwn = lidx(wbA + 1, wbB + 1);
     const int wbX = negX ? nx - 1 - (X0 + wbA) : X0 + wbA;
     const int wbY = negY ? ny - 1 - (Y0 + wbB) : Y0 + wbB;
-    const int64_t wbXY = (int64_t) wbY*nx + wbX;
+    const uint32_t wbB8 = (uint32_t) (wbY*nx + wbX)*sizeof(TS);
     // Steps: every plane passes every lane and is written back; rounded up
     // to whole groups of DL so the unrolled loop body has no exit (extra
     // steps only repeat idempotent work: clamped loads, no computation, and
@@ -443,12 +474,16 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             }
             // (b) prefetch plane tc + DL into the freed registers (past the
             // last plane the load repeats it and is never stored)
-            loadPlane(min(tc + DL, nPlanes - 1), bT0[u], bT1[u], bS0[u],
-                      bS1[u]);
+            loadPlane(loadT, loadS, bT0[u], bT1[u], bS0[u], bS1[u]);
+            if (loadQ < nPlanes - 1)
+            {
+                loadQ += 1;
+                loadT += stepT;
+                if (P0 + loadQ <= ncz - 1) { loadS += stepS; }
+            }
             // (d, read) the plane the last lane finished a step ago; read
             // with the neighbour values below, stored after the update
-            const int w = min(max(tc - (kColumnSkew + 1), 0), nPlanes - 1);
-            const TS wbValue = myT[w & (kColumnRing - 1)][wbOwn];
+            const TS wbValue = myT[wbQ & (kColumnRing - 1)][wbOwn];
             if (kGroup)
             {
                 // the group's slowness plane entered above is read below by
@@ -525,11 +560,11 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // unchanged nodes store their own value, lanes past the grid
             // edge repeat an edge lane's store), so the prefetch loads above
             // always have the same number of memory operations behind them.
-            if (active)
+            if (active) { *(TS *) (wbT + wbB8) = wbValue; }
+            if (tc >= kColumnSkew + 1 && wbQ < nPlanes - 1)
             {
-                const int jz = P0 + w;
-                const int az = negZ ? nz - 1 - jz : jz;
-                Tp[(int64_t) az*nxy + wbXY] = wbValue;
+                wbQ += 1;
+                wbT += stepT;
             }
             // LDS traffic of this step must not be reordered with the next
             // step's: other lanes of the wave consume it.  The fence covers
