@@ -170,12 +170,16 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
     del batch
 
     # The dominant kernel on its own: one more solve with eager launches,
-    # each bulk column launch bracketed by HIP events on its stream.
+    # each bulk column launch bracketed by HIP events on its stream.  One
+    # chain, so that no two launches overlap and each launch's duration is
+    # its own (each launch then covers one column diagonal of every
+    # problem); the timed solve above uses the library's chains.
     if not args.no_kernel_timing:
         topts = ekx.SolverOptions(number_of_sweeps=w["n_sweeps"],
                                   factored_radius=w["radius"],
                                   precision=getattr(ekx.Precision, mode),
-                                  kernel_timing=True, chains=args.chains,
+                                  kernel_timing=True,
+                                  chains=args.timing_chains,
                                   prefetch_planes=args.prefetch,
                                   chunk_planes=args.chunk,
                                   column_group=args.column_group,
@@ -293,6 +297,9 @@ def main():
                     help="first copy MIB MiB device-to-device 3 times (a "
                          "plain copy kernel of known size, the scale for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--timing-chains", type=int, default=1,
+                    help="launch chains of the per-launch timing solve "
+                         "(1: launches never overlap)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the eager per-launch timing solve (the "
                          "roofline then uses the whole sweep phase)")
