@@ -124,6 +124,14 @@ class _Problem3D(ctypes.Structure):
                 ("source", ctypes.c_double * 3)]
 
 
+class _LayeredGaussian(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int),
+                ("layer_velocity", ctypes.POINTER(ctypes.c_double)),
+                ("amplitude", ctypes.c_double),
+                ("correlation_length", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
 # Every symbol declared in include/eikonalxx_amd.h.
 EXPORTED_SYMBOLS = [
     "ekx_last_error", "ekx_version", "ekx_solver_options_default",
@@ -159,6 +167,7 @@ EXPORTED_SYMBOLS = [
     "ekx_solver2d_compute_travel_time_gradient_field",
     "ekx_solver2d_have_travel_time_gradient_field",
     "ekx_solver2d_get_travel_time_gradient_field",
+    "ekx_model3d_layered_gaussian", "ekx_solver3d_set_layered_gaussian_model",
 ]
 
 _lib = None
@@ -253,6 +262,11 @@ def _declare(L):
     L.ekx_solver2d_compute_travel_time_gradient_field.argtypes = [_P]
     L.ekx_solver2d_have_travel_time_gradient_field.argtypes = [_P]
     L.ekx_solver2d_get_travel_time_gradient_field.argtypes = [_P, _I64, _PD]
+    L.ekx_model3d_layered_gaussian.argtypes = [
+        ctypes.POINTER(_Geometry3D), ctypes.POINTER(_LayeredGaussian), _I64,
+        _PD]
+    L.ekx_solver3d_set_layered_gaussian_model.argtypes = [
+        _P, ctypes.POINTER(_LayeredGaussian)]
 
 
 def _xyz(locations):
@@ -392,6 +406,39 @@ class Geometry2D:
                            self.z0)
 
 
+class LayeredGaussianModel:
+    """A velocity model generated on the GPU (no reference counterpart):
+    len(layer_velocities) equal-thickness layers in z (top first), each
+    node's layer velocity times (1 + amplitude G), G a unit-variance
+    Gaussian random field with covariance ~ exp(-r^2 / l^2),
+    l = correlation_length (m), drawn from `seed`."""
+
+    def __init__(self, layer_velocities, amplitude=0.0,
+                 correlation_length=0.0, seed=0):
+        self.layer_velocities = np.ascontiguousarray(layer_velocities,
+                                                     dtype=np.float64)
+        self.amplitude = float(amplitude)
+        self.correlation_length = float(correlation_length)
+        self.seed = int(seed)
+
+    def _c(self):
+        v = self.layer_velocities
+        return _LayeredGaussian(v.size, _dp(v), self.amplitude,
+                                self.correlation_length,
+                                self.seed & 0xFFFFFFFFFFFFFFFF)
+
+    def generate(self, geometry):
+        """Nodal velocities, natural order (x fastest), as a flat array."""
+        L = library()
+        g = geometry._c()
+        m = self._c()
+        out = np.empty(geometry.n_grid)
+        _check(L.ekx_model3d_layered_gaussian(ctypes.byref(g),
+                                              ctypes.byref(m), out.size,
+                                              _dp(out)))
+        return out
+
+
 class Solver3D:
     """EikonalXX::Solver3D<double> (include/eikonalxx/solver3d.hpp:19-177)
     on the GPU."""
@@ -431,6 +478,13 @@ class Solver3D:
         v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
         _check(self._L.ekx_solver3d_set_cell_velocities(
             self._h, v.size, _dp(v), int(ordering)))
+
+    def set_layered_gaussian_model(self, model):
+        """Generates `model` (a LayeredGaussianModel) in device memory and
+        sets it as nodal velocities, with no host copy of the field."""
+        m = model._c()
+        _check(self._L.ekx_solver3d_set_layered_gaussian_model(
+            self._h, ctypes.byref(m)))
 
     def have_velocity_model(self):
         return bool(self._L.ekx_solver3d_have_velocity_model(self._h))

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libeikonalxx_amd.so")
 SOURCES = ["ekx_kernels3d.hip", "ekx_kernels2d.hip", "ekx_fields.hip",
-           "ekx_capi.hip", "ekx_capi2d.hip"]
+           "ekx_model3d.hip", "ekx_capi.hip", "ekx_capi2d.hip"]
 ARCH = os.environ.get("EKX_OFFLOAD_ARCH", "gfx950")
 
 

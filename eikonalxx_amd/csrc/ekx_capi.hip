@@ -1708,6 +1708,108 @@ struct Engine3D
     }
 };
 
+// ---------------------------------------------------------------------------
+// Generated layered Gaussian models (ekx_layered_gaussian)
+// ---------------------------------------------------------------------------
+ekx_status check_layered_gaussian(const ekx_layered_gaussian *m)
+{
+    if (!m) { return fail(EKX_EINVAL, "model is NULL"); }
+    if (m->n_layers < 1 || !m->layer_velocity)
+    {
+        return fail(EKX_EINVAL, "a model needs at least one layer velocity");
+    }
+    for (int k = 0; k < m->n_layers; ++k)
+    {
+        if (!(m->layer_velocity[k] > 0) || !std::isfinite(m->layer_velocity[k]))
+        {
+            return fail(EKX_EINVAL, "Non-positive velocities detected");
+        }
+    }
+    if (!(m->amplitude >= 0) || !std::isfinite(m->amplitude))
+    {
+        return fail(EKX_EINVAL, "amplitude must be >= 0");
+    }
+    if (!(m->correlation_length >= 0) ||
+        !std::isfinite(m->correlation_length))
+    {
+        return fail(EKX_EINVAL, "correlation length must be >= 0");
+    }
+    return EKX_OK;
+}
+
+// Nodal velocities of the model into device memory dV (nx*ny*nz values,
+// natural order), on stream s (synchronised on return).
+ekx_status generate_layered_gaussian(const ekx_geometry3d &g,
+                                     const ekx_layered_gaussian &m,
+                                     double *dV, hipStream_t s)
+{
+    const int64_t n = (int64_t) g.nx*g.ny*g.nz;
+    DeviceBuffer<double> a, b, w, partial, layers;
+    EKX_CHECK(a.allocate((size_t) n));
+    EKX_CHECK(b.allocate((size_t) n));
+    EKX_CHECK(partial.allocate(kMaxPartials));
+    EKX_CHECK(layers.allocate((size_t) m.n_layers));
+    EKX_HIP(hipMemcpyAsync(layers.ptr, m.layer_velocity,
+                           (size_t) m.n_layers*sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    EKX_HIP(launch_gaussian_noise(n, m.seed, a.ptr, s));
+    const double d[3] = {g.dx, g.dy, g.dz};
+    std::vector<double> taps;
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        // sigma = l / (2 d) nodes, radius int(4 sigma + 0.5)
+        const double sigma = m.correlation_length/(2.0*d[axis]);
+        const int radius = (int) (4.0*sigma + 0.5);
+        if (radius < 1) { continue; }
+        taps.assign((size_t) (2*radius + 1), 0.0);
+        double sum = 0.0;
+        for (int k = -radius; k <= radius; ++k)
+        {
+            const double x = k/sigma;
+            taps[(size_t) (k + radius)] = std::exp(-0.5*x*x);
+            sum += taps[(size_t) (k + radius)];
+        }
+        for (double &t : taps) { t /= sum; }
+        EKX_CHECK(w.allocate(taps.size()));
+        EKX_HIP(hipMemcpyAsync(w.ptr, taps.data(), taps.size()*sizeof(double),
+                               hipMemcpyHostToDevice, s));
+        EKX_HIP(launch_filter_axis(g.nx, g.ny, g.nz, axis, radius, w.ptr,
+                                   a.ptr, b.ptr, s));
+        EKX_HIP(hipStreamSynchronize(s));    // taps are reused per axis
+        std::swap(a.ptr, b.ptr);
+    }
+    std::vector<double> host(kMaxPartials);
+    auto reduce = [&](const double *v, double mean, int power,
+                      double &out) -> ekx_status
+    {
+        EKX_HIP(launch_moment(n, v, mean, power, partial.ptr, kMaxPartials,
+                              s));
+        EKX_HIP(hipMemcpyAsync(host.data(), partial.ptr,
+                               kMaxPartials*sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+        EKX_HIP(hipStreamSynchronize(s));
+        out = power == 0 ? host[0] : 0.0;
+        for (double x : host) { out = power == 0 ? std::min(out, x) : out + x; }
+        return EKX_OK;
+    };
+    double sum = 0, ss = 0;
+    EKX_CHECK(reduce(a.ptr, 0.0, 1, sum));
+    const double mean = sum/(double) n;
+    EKX_CHECK(reduce(a.ptr, mean, 2, ss));
+    double sd = std::sqrt(ss/(double) n);
+    if (!(sd > 0)) { sd = 1.0; }
+    EKX_HIP(launch_layered_velocity(g.nx, g.ny, g.nz, m.n_layers, layers.ptr,
+                                    m.amplitude, mean, sd, a.ptr, dV, s));
+    double vmin = 0;
+    EKX_CHECK(reduce(dV, 0.0, 0, vmin));
+    if (!(vmin > 0))
+    {
+        return fail(EKX_EINVAL, "Non-positive velocities detected (the "
+                                "amplitude is too large for the field)");
+    }
+    return EKX_OK;
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1827,6 +1929,58 @@ ekx_status ekx_solver3d_set_slowness(ekx_solver3d *h, int64_t n,
     }
     EKX_CHECK(h->engine.upload_model(0, slowness, n));
     return h->engine.finish_models();
+}
+
+ekx_status ekx_model3d_layered_gaussian(const ekx_geometry3d *geometry,
+                                        const ekx_layered_gaussian *model,
+                                        int64_t n, double *velocity)
+{
+    EKX_CHECK(check_geometry3d(geometry));
+    EKX_CHECK(check_layered_gaussian(model));
+    const ekx_geometry3d &g = *geometry;
+    const int64_t nGrid = (int64_t) g.nx*g.ny*g.nz;
+    if (n != nGrid || !velocity)
+    {
+        return fail(EKX_EINVAL, "output must hold the " +
+                                    std::to_string(nGrid) + " grid points");
+    }
+    hipStream_t s = nullptr;
+    EKX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DeviceBuffer<double> dv;
+    ekx_status rc = dv.allocate((size_t) n);
+    if (rc == EKX_OK) { rc = generate_layered_gaussian(g, *model, dv.ptr, s); }
+    if (rc == EKX_OK)
+    {
+        hipError_t e = hipMemcpy(velocity, dv.ptr, (size_t) n*sizeof(double),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+        {
+            rc = fail(EKX_EHIP, std::string("hipMemcpy: ") +
+                                    hipGetErrorString(e));
+        }
+    }
+    (void) hipStreamDestroy(s);
+    return rc;
+}
+
+ekx_status ekx_solver3d_set_layered_gaussian_model(
+    ekx_solver3d *h, const ekx_layered_gaussian *model)
+{
+    EKX_CHECK(require_initialized(h));
+    h->engine.haveModel = false;
+    h->engine.haveField = false;
+    EKX_CHECK(check_layered_gaussian(model));
+    const ekx_geometry3d &g = h->geometry;
+    Engine3D &e = h->engine;
+    e.drop_graph();
+    DeviceBuffer<double> dv;
+    EKX_CHECK(dv.allocate((size_t) g.nx*g.ny*g.nz));
+    EKX_CHECK(generate_layered_gaussian(g, *model, dv.ptr, e.stream));
+    EKX_HIP(launch_nodal_to_cell3d(g.nx, g.ny, g.nz, dv.ptr,
+                                   EKX_ORDERING_NATURAL, e.S64.ptr,
+                                   e.stream));
+    EKX_HIP(hipStreamSynchronize(e.stream));
+    return e.finish_models();
 }
 
 ekx_status ekx_solver3d_set_nodal_velocities(ekx_solver3d *h, int64_t n,

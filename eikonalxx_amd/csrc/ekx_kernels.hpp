@@ -116,6 +116,23 @@ hipError_t launch_gradient2d(const Problem2D *dProblem, int nx, int nz,
                              const double *T, double *grad,
                              hipStream_t stream);
 
+// Generated models (ekx_model3d.hip): Gaussian white noise from (seed,
+// node index); one axis of a separable filter with 2*radius+1 taps and
+// half-sample-symmetric boundaries; block partial sums (power 1), sums of
+// squared deviations from mean (power 2) or minima (power 0); layered
+// velocity times (1 + amplitude (g - mean) / sd).
+hipError_t launch_gaussian_noise(int64_t n, uint64_t seed, double *out,
+                                 hipStream_t stream);
+hipError_t launch_filter_axis(int nx, int ny, int nz, int axis, int radius,
+                              const double *w, const double *in, double *out,
+                              hipStream_t stream);
+hipError_t launch_moment(int64_t n, const double *v, double mean, int power,
+                         double *partial, int nPartial, hipStream_t stream);
+hipError_t launch_layered_velocity(int nx, int ny, int nz, int nLayers,
+                                   const double *layerV, double amplitude,
+                                   double mean, double sd, const double *g,
+                                   double *v, hipStream_t stream);
+
 hipError_t launch_solve2d(const Problem2D *dProblems, int nProblems,
                           double *T, const double *S, hipStream_t stream);
 hipError_t launch_stations2d(const Problem2D *dProblem, int n,

@@ -305,6 +305,36 @@ EKX_API ekx_status ekx_solver3d_get_stats(const ekx_solver3d *solver,
 EKX_API ekx_status ekx_solver3d_get_stream(const ekx_solver3d *solver,
                                            void **stream);
 
+/* ------------------------- Generated models ----------------------------
+ * No reference counterpart (the reference reads models from the caller): a
+ * velocity model of n_layers equal-thickness layers in z (layer k holds
+ * nodes iz with floor(iz n_layers / nz) = k), each node's layer velocity
+ * times (1 + amplitude G), G a unit-variance Gaussian random field with
+ * covariance ~ exp(-r^2 / l^2): white noise from (seed, node index) through
+ * a splitmix64 counter hash and the Box-Muller transform, filtered per axis
+ * by a Gaussian of sigma = l / (2 d) nodes truncated at 4 sigma with
+ * half-sample-symmetric boundaries (x, then y, then z), shifted and scaled
+ * to zero mean and unit population variance.  Generated on the GPU. */
+typedef struct
+{
+    int n_layers;
+    const double *layer_velocity;  /* host, n_layers values (m/s), top first */
+    double amplitude;              /* relative perturbation, >= 0          */
+    double correlation_length;     /* l in m; 0: white noise               */
+    uint64_t seed;
+} ekx_layered_gaussian;
+
+/* Nodal velocities (natural order, nx*ny*nz values) into host memory.
+ * EKX_EINVAL if a velocity would be <= 0. */
+EKX_API ekx_status ekx_model3d_layered_gaussian(
+    const ekx_geometry3d *geometry, const ekx_layered_gaussian *model,
+    int64_t n_grid, double *velocity);
+/* Generates the model in device memory and sets it as the solver's model
+ * (nodal velocities to cell slowness on the device, src/model3d.cpp:45-151),
+ * with no host copy of the field. */
+EKX_API ekx_status ekx_solver3d_set_layered_gaussian_model(
+    ekx_solver3d *solver, const ekx_layered_gaussian *model);
+
 /* ------------------------------ Batch3D --------------------------------
  * Many independent 3D solves in one schedule of launches (no reference
  * counterpart: the reference solves one source per Solver3D).  A problem

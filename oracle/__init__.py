@@ -82,6 +82,13 @@ def _declare(L):
     for name, geo in (("ekxo_gradient2d", Geometry2D),
                       ("ekxo_gradient3d", Geometry3D)):
         getattr(L, name).restype = _I
+    L.ekxo_gaussian_noise.restype = None
+    L.ekxo_gaussian_noise.argtypes = [ctypes.c_int64, ctypes.c_uint64, _PD]
+    L.ekxo_gaussian_filter3d.restype = _I
+    L.ekxo_gaussian_filter3d.argtypes = [ctypes.POINTER(Geometry3D), _D, _PD]
+    L.ekxo_layered_gaussian3d.restype = _I
+    L.ekxo_layered_gaussian3d.argtypes = [ctypes.POINTER(Geometry3D), _I, _PD,
+                                          _D, _D, ctypes.c_uint64, _PD]
     L.ekxo_station_time2d.restype = _D
     L.ekxo_station_time2d.argtypes = [ctypes.POINTER(Geometry2D), _PD, _PD,
                                       _D, _D, _D, _D]
@@ -486,4 +493,33 @@ def gradient3d(g, slowness, T, source):
     if rc != 0:
         raise ValueError("source outside the model" if rc == -1
                          else "at least 3 grid points per axis")
+    return out
+
+
+def gaussian_noise(n, seed):
+    """Unit normal noise of the generated models, one value per node."""
+    out = np.empty(int(n))
+    lib().ekxo_gaussian_noise(out.size, seed & 0xFFFFFFFFFFFFFFFF, _dp(out))
+    return out
+
+
+def gaussian_filter3d(g, correlation_length, field):
+    """The models' separable Gaussian filter of a natural-order field."""
+    a = np.array(field, dtype=np.float64).reshape(-1)
+    if lib().ekxo_gaussian_filter3d(ctypes.byref(g), _D(correlation_length),
+                                    _dp(a)) != 0:
+        raise MemoryError("filter workspace")
+    return a
+
+
+def layered_gaussian3d(g, layer_velocities, amplitude, correlation_length,
+                       seed):
+    """Nodal velocities (natural order) of a generated layered model."""
+    lv = np.ascontiguousarray(layer_velocities, dtype=np.float64)
+    out = np.empty(g.nx * g.ny * g.nz)
+    if lib().ekxo_layered_gaussian3d(ctypes.byref(g), lv.size, _dp(lv),
+                                     _D(amplitude), _D(correlation_length),
+                                     seed & 0xFFFFFFFFFFFFFFFF,
+                                     _dp(out)) != 0:
+        raise MemoryError("model workspace")
     return out

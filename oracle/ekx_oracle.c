@@ -1845,3 +1845,121 @@ int ekxo_gradient3d(const ekxo_geometry3d *g, const double *slow,
     }
     return 0;
 }
+
+/* ======================================================================== */
+/*                     Generated layered Gaussian models                    */
+/* ======================================================================== */
+
+/* The generator of ekx_layered_gaussian (a build extension; the reference
+ * reads models from the caller): splitmix64 counter hash -> two uniforms
+ * -> Box-Muller normal per node; separable Gaussian filter per axis (x, y,
+ * z), sigma = l / (2 d) nodes, radius int(4 sigma + 0.5), taps
+ * exp(-x^2/2) normalised, half-sample-symmetric boundaries, taps summed in
+ * increasing order; shift and scale to zero mean and unit population
+ * variance; layer velocity of floor(iz nLayers / nz) times (1 + a G). */
+static uint64_t ekxo_mix64(uint64_t x)
+{
+    x = (x ^ (x >> 30))*0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27))*0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+static double ekxo_uniform01(uint64_t seed, uint64_t k)
+{
+    const uint64_t z = ekxo_mix64(seed + (k + 1)*0x9E3779B97F4A7C15ull);
+    return ((double) (z >> 11) + 0.5)*0x1p-53;
+}
+
+static int ekxo_reflect(int i, int n)
+{
+    const int period = 2*n;
+    i %= period;
+    if (i < 0) { i += period; }
+    return i < n ? i : period - 1 - i;
+}
+
+void ekxo_gaussian_noise(int64_t n, uint64_t seed, double *out)
+{
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const double u1 = ekxo_uniform01(seed, 2*(uint64_t) i);
+        const double u2 = ekxo_uniform01(seed, 2*(uint64_t) i + 1);
+        out[i] = sqrt(-2.0*log(u1))*cos(6.283185307179586476925*u2);
+    }
+}
+
+/* The filtered (not yet normalised) field of the noise in `a`, in place. */
+int ekxo_gaussian_filter3d(const ekxo_geometry3d *g, double l, double *a)
+{
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const double d[3] = {g->dx, g->dy, g->dz};
+    const int64_t nxy = (int64_t) g->nx*g->ny;
+    const int64_t total = nxy*g->nz;
+    const int64_t stride[3] = {1, g->nx, nxy};
+    double *b = (double *) malloc((size_t) total*sizeof(double));
+    if (!b) { return -1; }
+    for (int axis = 0; axis < 3; ++axis)
+    {
+        const double sigma = l/(2.0*d[axis]);
+        const int radius = (int) (4.0*sigma + 0.5);
+        if (radius < 1) { continue; }
+        double *w = (double *) malloc((size_t) (2*radius + 1)*sizeof(double));
+        if (!w) { free(b); return -1; }
+        double sum = 0.0;
+        for (int k = -radius; k <= radius; ++k)
+        {
+            const double x = k/sigma;
+            w[k + radius] = exp(-0.5*x*x);
+            sum += w[k + radius];
+        }
+        for (int k = 0; k <= 2*radius; ++k) { w[k] /= sum; }
+        for (int64_t i = 0; i < total; ++i)
+        {
+            const int iz = (int) (i/nxy);
+            const int rem = (int) (i - (int64_t) iz*nxy);
+            const int iy = rem/g->nx;
+            const int ix = rem - iy*g->nx;
+            const int pos = axis == 0 ? ix : (axis == 1 ? iy : iz);
+            const int64_t base = i - (int64_t) pos*stride[axis];
+            double acc = 0.0;
+            for (int k = 0; k <= 2*radius; ++k)
+            {
+                const int j = ekxo_reflect(pos + k - radius, n[axis]);
+                acc += w[k]*a[base + (int64_t) j*stride[axis]];
+            }
+            b[i] = acc;
+        }
+        memcpy(a, b, (size_t) total*sizeof(double));
+        free(w);
+    }
+    free(b);
+    return 0;
+}
+
+int ekxo_layered_gaussian3d(const ekxo_geometry3d *g, int nLayers,
+                            const double *layerV, double amplitude, double l,
+                            uint64_t seed, double *v)
+{
+    const int64_t nxy = (int64_t) g->nx*g->ny;
+    const int64_t n = nxy*g->nz;
+    double *a = (double *) malloc((size_t) n*sizeof(double));
+    if (!a) { return -1; }
+    ekxo_gaussian_noise(n, seed, a);
+    if (ekxo_gaussian_filter3d(g, l, a) != 0) { free(a); return -1; }
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; ++i) { sum += a[i]; }
+    const double mean = sum/(double) n;
+    double ss = 0.0;
+    for (int64_t i = 0; i < n; ++i) { ss += (a[i] - mean)*(a[i] - mean); }
+    double sd = sqrt(ss/(double) n);
+    if (!(sd > 0)) { sd = 1.0; }
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const int iz = (int) (i/nxy);
+        int layer = (int) (((int64_t) iz*nLayers)/g->nz);
+        if (layer > nLayers - 1) { layer = nLayers - 1; }
+        v[i] = layerV[layer]*(1.0 + amplitude*((a[i] - mean)/sd));
+    }
+    free(a);
+    return 0;
+}

@@ -34,6 +34,10 @@
  *       linear gradient 1e-4 against the analytic gradients); exact on
  *       quadratics (every edge stencil); 3D equals 2D on y-invariant
  *       fields; the source-corner values against the closed form.
+ *   generated models (no reference counterpart) -> the filter against
+ *       scipy.ndimage.gaussian_filter (mode 'reflect', truncate 4) on the
+ *       same noise, the noise's moments and lag correlation, the hash
+ *       against an independent integer restatement, the layer boundaries.
  *   Bitwise parity with the reference binary itself is unpinned: the
  *   reference needs SYCL and TBB headers that this image lacks.
  */
@@ -204,6 +208,13 @@ int ekxo_gradient2d(const ekxo_geometry2d *g, const double *slow,
 int ekxo_gradient3d(const ekxo_geometry3d *g, const double *slow,
                     const double *T, double xs, double ys, double zs,
                     double *grad);
+/* Generated layered Gaussian models (the generator of the library's
+ * ekx_layered_gaussian): white noise, its Gaussian filter, the model. */
+void ekxo_gaussian_noise(int64_t n, uint64_t seed, double *out);
+int ekxo_gaussian_filter3d(const ekxo_geometry3d *g, double l, double *a);
+int ekxo_layered_gaussian3d(const ekxo_geometry3d *g, int nLayers,
+                            const double *layerV, double amplitude, double l,
+                            uint64_t seed, double *v);
 /* Bilinear travel time at a point (src/solver2d.cpp:106-167). */
 double ekxo_station_time2d(const ekxo_geometry2d *g, const double *slow,
                            const double *T, double xs, double zs, double xr,
