@@ -122,8 +122,6 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                              chains=args.chains,
                              prefetch_planes=args.prefetch,
                              chunk_planes=args.chunk,
-                             column_group=args.column_group,
-                             sweep_overlap=args.overlap,
                              use_graph=not args.no_graph)
     batch = ekx.Batch3D()
     batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
@@ -181,9 +179,7 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                                   kernel_timing=True,
                                   chains=args.timing_chains,
                                   prefetch_planes=args.prefetch,
-                                  chunk_planes=args.chunk,
-                                  column_group=args.column_group,
-                                  sweep_overlap=args.overlap)
+                                  chunk_planes=args.chunk)
         tb = ekx.Batch3D()
         tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
         tb.solve()
@@ -277,12 +273,6 @@ def main():
     ap.add_argument("--chunk", type=int, default=0,
                     help="z-planes per column chunk (0: the library's "
                          "choice)")
-    ap.add_argument("--column-group", type=int, default=0,
-                    help="1: grouped refinement launches for batches on "
-                         "one model (0: one problem per wave)")
-    ap.add_argument("--overlap", type=int, default=0,
-                    help="1: each sweep starts while the previous one "
-                         "finishes (0: sweep after sweep)")
     ap.add_argument("--chains", type=int, default=0,
                     help="independent launch chains (0: the library's "
                          "choice)")

@@ -79,9 +79,7 @@ class _Options(ctypes.Structure):
                 ("kernel_timing", ctypes.c_int),
                 ("chains", ctypes.c_int),
                 ("prefetch_planes", ctypes.c_int),
-                ("chunk_planes", ctypes.c_int),
-                ("column_group", ctypes.c_int),
-                ("sweep_overlap", ctypes.c_int)]
+                ("chunk_planes", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -107,7 +105,7 @@ class SolveStats(ctypes.Structure):
                 ("timed_launches", ctypes.c_int64),
                 ("timed_updates", ctypes.c_int64),
                 ("iterations_run", ctypes.c_int32),
-                ("group_waves", ctypes.c_int32),
+                ("tiled", ctypes.c_int32),
                 ("timed_kernel_ms", ctypes.c_double),
                 ("last_iteration_change", ctypes.c_double),
                 ("last_solve_ms", ctypes.c_double),
@@ -309,8 +307,7 @@ class SolverOptions:
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
                  use_graph=True, verbosity=0, kernel_timing=False,
-                 chains=0, prefetch_planes=0, chunk_planes=0,
-                 column_group=0, sweep_overlap=0):
+                 chains=0, prefetch_planes=0, chunk_planes=0):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -330,8 +327,6 @@ class SolverOptions:
         o.chains = int(chains)
         o.prefetch_planes = int(prefetch_planes)
         o.chunk_planes = int(chunk_planes)
-        o.column_group = int(column_group)
-        o.sweep_overlap = int(sweep_overlap)
         o.verbosity = int(verbosity)
         self._o = o
 

@@ -162,14 +162,6 @@ ekx_status check_options(const ekx_solver_options *o)
     {
         return fail(EKX_EINVAL, "chunk_planes must be >= 0");
     }
-    if (o->column_group != 0 && o->column_group != 1)
-    {
-        return fail(EKX_EINVAL, "column_group must be 0 or 1");
-    }
-    if (o->sweep_overlap != 0 && o->sweep_overlap != 1)
-    {
-        return fail(EKX_EINVAL, "sweep_overlap must be 0 or 1");
-    }
     if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
         o->prefetch_planes != 8 && o->prefetch_planes != 12)
     {
@@ -300,21 +292,28 @@ int64_t host_box_updates(const Problem3D &p, int sweep, bool initialize,
     int zLo = std::max(p.srcCellZ - p.radius, 0);
     int zHi = std::min(p.srcCellZ + p.radius, p.nz - 1);
     if (xLo > xHi || yLo > yHi || zLo > zHi) { return 0; }
-    int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
-    int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
     int jzLo = negZ ? p.nz - 1 - zHi : zLo, jzHi = negZ ? p.nz - 1 - zLo : zHi;
     const HostOctant o = host_octant(p, sweep, initialize);
-    // nodes of blocks bLo..bHi (w wide) at or past j0, below n
+    // permuted nodes of the tiles holding grid nodes lo..hi (tiles are
+    // aligned to the grid: tile t holds 8 t .. 8 t + 7), at or past j0
+    auto tileSpan = [](int lo, int hi, int neg, int j0, int n)
+    {
+        const int tLo = lo/kColumnTile, tHi = hi/kColumnTile;
+        const int gLo = tLo*kColumnTile;
+        const int gHi = std::min(tHi*kColumnTile + kColumnTile - 1, n - 1);
+        const int jLo = std::max(neg ? n - 1 - gHi : gLo, j0);
+        const int jHi = neg ? n - 1 - gLo : gHi;
+        return std::max(0, jHi - jLo + 1);
+    };
+    // nodes of chunks bLo..bHi (w wide) at or past j0, below n
     auto span = [](int bLo, int bHi, int w, int j0, int n)
     {
         int lo = std::max(bLo*w, j0);
         int hi = std::min(bHi*w + w - 1, n - 1);
         return std::max(0, hi - lo + 1);
     };
-    int64_t nxs = span(jxLo/kColumnTile, jxHi/kColumnTile, kColumnTile,
-                       o.jx0, p.nx);
-    int64_t nys = span(jyLo/kColumnTile, jyHi/kColumnTile, kColumnTile,
-                       o.jy0, p.ny);
+    int64_t nxs = tileSpan(xLo, xHi, negX, o.jx0, p.nx);
+    int64_t nys = tileSpan(yLo, yHi, negY, o.jy0, p.ny);
     int64_t nzs = span(jzLo/chunk, jzHi/chunk, chunk, o.jz0, p.nz);
     return nxs*nys*nzs;
 }
@@ -340,6 +339,13 @@ struct Engine3D
     std::vector<int64_t> modelOffset;
     int64_t totalNodes = 0, totalCells = 0, maxNodes = 0;
     std::vector<int64_t> modelCells;
+    // Fields are tiled for the column kernel (tile records and edge
+    // records per problem, tCount elements from tOffset) and natural for
+    // the hyperplane kernel.  Models: natural cells (S*), and for the
+    // column kernel also slowness bricks (SB*, from sbOffset per model).
+    bool tiled = true;
+    int64_t maxCount = 0, totalBricks = 0;
+    std::vector<int64_t> modelBrickOffset;
     DeviceBuffer<double> S64;
     DeviceBuffer<float> S32;
     DeviceBuffer<double> T64;
@@ -347,6 +353,15 @@ struct Engine3D
     // scaled-integer storage (FIXED32 modes)
     DeviceBuffer<uint32_t> SQ;
     DeviceBuffer<uint32_t> TQ;
+    DeviceBuffer<double> SB64;
+    DeviceBuffer<float> SB32;
+    DeviceBuffer<uint32_t> SBQ;
+    // natural-order copy of one problem's field (device pointer access)
+    mutable DeviceBuffer<double> natural64;
+    mutable DeviceBuffer<float> natural32;
+    mutable DeviceBuffer<uint32_t> naturalQ;
+    mutable int naturalProblem = -1;
+    mutable uint64_t naturalStamp = 0;
     DeviceBuffer<double> partial;
     std::vector<double> modelSMax;   // largest slowness of each model
     DeviceBuffer<Problem3D> dProblems;
@@ -361,31 +376,6 @@ struct Engine3D
     std::vector<int> chainBegin;
     std::vector<hipStream_t> chainStreams;
     std::vector<hipEvent_t> chainEvents;
-    // Sweep overlap: per chain a second stream (odd sweeps of a list), its
-    // end event, and the events recorded after each diagonal of the last
-    // two sweeps ([chain][parity][diagonal]); reqDiag[(c*8 + k)*nDiag + d]
-    // is the last diagonal of sweep k - 1 (mod 8) that must be complete
-    // before chain c's sweep k starts its diagonal d.
-    std::vector<hipStream_t> chainStreams2;
-    std::vector<hipEvent_t> chainEvents2;
-    std::vector<hipEvent_t> diagEvents;
-    std::vector<int> reqDiag;
-    bool overlap() const
-    {
-        return opt.sweep_overlap == 1 && kernel() == EKX_KERNEL_COLUMN;
-    }
-    // Grouped refinement sweeps (every problem on one model and one
-    // geometry): per group size G (column_group<TS>() of the element types
-    // in use), items {first problem of a group of G, bx, by, bz} ordered by
-    // chain, then diagonal, as diagStart orders the single items.
-    struct GroupItems
-    {
-        int G = 0;
-        DeviceBuffer<ColumnItem> items;
-        std::vector<int> diagStart;
-    };
-    GroupItems groupItems[2];
-    bool grouped = false;
     // Upper bound, over any source position, of the columns per diagonal
     // that overlap one problem's factored box: the box spans at most
     // ceil((2r+1)/8)+1 columns per axis.
@@ -455,10 +445,7 @@ struct Engine3D
         if (evJoin) { (void) hipEventDestroy(evJoin); }
         for (hipEvent_t &e : timingEvents) { (void) hipEventDestroy(e); }
         for (hipEvent_t &e : chainEvents) { (void) hipEventDestroy(e); }
-        for (hipEvent_t &e : chainEvents2) { (void) hipEventDestroy(e); }
-        for (hipEvent_t &e : diagEvents) { (void) hipEventDestroy(e); }
         for (hipStream_t &c : chainStreams) { (void) hipStreamDestroy(c); }
-        for (hipStream_t &c : chainStreams2) { (void) hipStreamDestroy(c); }
         if (changeHost) { (void) hipHostFree(changeHost); }
         if (sideStream) { (void) hipStreamDestroy(sideStream); }
         for (hipEvent_t &e : evSeg)
@@ -552,12 +539,28 @@ struct Engine3D
                 totalCells += modelCells[m];
             }
         }
+        tiled = kernel() == EKX_KERNEL_COLUMN;
+        modelBrickOffset.assign(nModels, -1);
+        totalBricks = 0;
         totalNodes = 0;
         maxNodes = 0;
+        maxCount = 0;
         for (int i = 0; i < n; ++i)
         {
             const ekx_geometry3d &g = geoms[i];
             Problem3D &p = problems[i];
+            p.tiled = tiled ? 1 : 0;
+            p.nTX = tiles_of(g.nx);
+            p.nTY = tiles_of(g.ny);
+            p.pad = 0;
+            const int64_t nTiles = (int64_t) p.nTX*p.nTY;
+            const int m = modelOf[i];
+            if (modelBrickOffset[m] < 0)
+            {
+                modelBrickOffset[m] = totalBricks;
+                totalBricks += nTiles*(g.nz - 1)*kBrickValues;
+            }
+            p.sbOffset = modelBrickOffset[m];
             p.nx = g.nx;
             p.ny = g.ny;
             p.nz = g.nz;
@@ -570,13 +573,30 @@ struct Engine3D
             p.sOffset = modelOffset[modelOf[i]];
             p.nu = non_uniform_constants(g);
             int64_t nodes = (int64_t) g.nx*g.ny*g.nz;
-            totalNodes += nodes;
+            p.tCount = tiled ? nTiles*g.nz*(kTileValues + kEdgeValues) : nodes;
+            p.eOffset = p.tOffset + (tiled ? nTiles*g.nz*kTileValues : 0);
+            totalNodes += p.tCount;
             maxNodes = std::max(maxNodes, nodes);
+            maxCount = std::max(maxCount, p.tCount);
         }
+        naturalProblem = -1;
         EKX_CHECK(S64.allocate((size_t) totalCells));
         const bool fp32 = !sweeps_f64() && !fixed32();
         if (fp32) { EKX_CHECK(S32.allocate((size_t) totalCells)); }
         else { S32.release(); }
+        // bricks of the element types the sweeps read
+        if (tiled && (sweeps_f64() || polish()))
+        {
+            EKX_CHECK(SB64.allocate((size_t) totalBricks));
+        }
+        else { SB64.release(); }
+        if (tiled && fp32) { EKX_CHECK(SB32.allocate((size_t) totalBricks)); }
+        else { SB32.release(); }
+        if (tiled && fixed32())
+        {
+            EKX_CHECK(SBQ.allocate((size_t) totalBricks));
+        }
+        else { SBQ.release(); }
         if (fixed32())
         {
             EKX_CHECK(SQ.allocate((size_t) totalCells));
@@ -712,205 +732,8 @@ struct Engine3D
         EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
                                flat.size()*sizeof(ColumnItem),
                                hipMemcpyHostToDevice, stream));
-        EKX_CHECK(setup_groups(geoms, nModels));
-        if (overlap()) { EKX_CHECK(setup_overlap(geoms)); }
         EKX_HIP(hipStreamSynchronize(stream));
         return EKX_OK;
-    }
-
-    // Dependencies between consecutive sweeps (k - 1, k) for the sweep
-    // overlap.  A column chunk C of sweep k may start once every chunk of
-    // sweep k - 1 that writes a node C reads (its nodes and upwind halo)
-    // or reads a node C writes (the downwind neighbours of its nodes) has
-    // finished: all of them lie in C's node box grown by one node in every
-    // direction, and the last of them in sweep k - 1's order is on the
-    // diagonal (max column x) + (max column y) + (max chunk z) of that box
-    // in sweep k - 1's frame.
-    ekx_status setup_overlap(const std::vector<ekx_geometry3d> &geoms)
-    {
-        reqDiag.assign((size_t) nChains*8*nDiag, -1);
-        // natural index range of permuted [j0, j1] on an axis of n nodes,
-        // grown by one node and clipped
-        auto grown = [](int j0, int j1, int n, int neg, int &lo, int &hi)
-        {
-            j0 = std::max(j0, 0);
-            j1 = std::min(j1, n - 1);
-            lo = neg ? n - 1 - j1 : j0;
-            hi = neg ? n - 1 - j0 : j1;
-            lo = std::max(lo - 1, 0);
-            hi = std::min(hi + 1, n - 1);
-        };
-        // largest tile index of natural [lo, hi] in a frame
-        auto lastTile = [](int lo, int hi, int n, int neg, int tile)
-        {
-            return (neg ? n - 1 - lo : hi)/tile;
-        };
-        for (int c = 0; c < nChains; ++c)
-        {
-            for (int i = chainBegin[c]; i < chainBegin[c + 1]; ++i)
-            {
-                const ekx_geometry3d &g = geoms[i];
-                if (i > chainBegin[c])
-                {
-                    const ekx_geometry3d &h = geoms[i - 1];
-                    if (g.nx == h.nx && g.ny == h.ny && g.nz == h.nz)
-                    {
-                        continue;      // same tiles as the previous problem
-                    }
-                }
-                const int nbx = (g.nx + kColumnTile - 1)/kColumnTile;
-                const int nby = (g.ny + kColumnTile - 1)/kColumnTile;
-                const int nbz = (g.nz + chunkH - 1)/chunkH;
-                for (int k = 0; k < 8; ++k)
-                {
-                    const int kp = (k + 7)%8;
-                    const int nx0 = k & 1, ny0 = (k >> 1) & 1;
-                    const int nz0 = (k >> 2) & 1;
-                    const int nx1 = kp & 1, ny1 = (kp >> 1) & 1;
-                    const int nz1 = (kp >> 2) & 1;
-                    int *req = reqDiag.data() + ((size_t) c*8 + k)*nDiag;
-                    for (int bz = 0; bz < nbz; ++bz)
-                    {
-                        int zl, zh;
-                        grown(bz*chunkH - 1, (bz + 1)*chunkH - 1, g.nz, nz0,
-                              zl, zh);
-                        const int tz = lastTile(zl, zh, g.nz, nz1, chunkH);
-                        for (int by = 0; by < nby; ++by)
-                        {
-                            int yl, yh;
-                            grown(by*kColumnTile, by*kColumnTile + 7, g.ny,
-                                  ny0, yl, yh);
-                            const int ty = lastTile(yl, yh, g.ny, ny1,
-                                                    kColumnTile);
-                            for (int bx = 0; bx < nbx; ++bx)
-                            {
-                                int xl, xh;
-                                grown(bx*kColumnTile, bx*kColumnTile + 7,
-                                      g.nx, nx0, xl, xh);
-                                const int tx = lastTile(xl, xh, g.nx, nx1,
-                                                        kColumnTile);
-                                int &r = req[bx + by + bz];
-                                r = std::max(r, std::min(tx + ty + tz,
-                                                         nDiag - 1));
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        while (chainStreams2.size() < chainStreams.size())
-        {
-            hipStream_t s2 = nullptr;
-            hipEvent_t e2 = nullptr;
-            EKX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-            EKX_HIP(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
-            chainStreams2.push_back(s2);
-            chainEvents2.push_back(e2);
-        }
-        const size_t nEv = (size_t) nChains*2*nDiag;
-        while (diagEvents.size() < nEv)
-        {
-            hipEvent_t e = nullptr;
-            EKX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            diagEvents.push_back(e);
-        }
-        return EKX_OK;
-    }
-
-    // Group work lists, when every problem shares the one model and the
-    // geometry and every chain holds at least one full group.
-    ekx_status setup_groups(const std::vector<ekx_geometry3d> &geoms,
-                            int nModels)
-    {
-        grouped = false;
-        for (GroupItems &g : groupItems)
-        {
-            g.G = 0;
-            g.items.release();
-            g.diagStart.clear();
-        }
-        const int n = (int) geoms.size();
-        if (opt.column_group != 1 || kernel() != EKX_KERNEL_COLUMN ||
-            nModels != 1)
-        {
-            return EKX_OK;
-        }
-        const ekx_geometry3d &g0 = geoms[0];
-        for (const ekx_geometry3d &g : geoms)
-        {
-            if (g.nx != g0.nx || g.ny != g0.ny || g.nz != g0.nz ||
-                g.dx != g0.dx || g.dy != g0.dy || g.dz != g0.dz)
-            {
-                return EKX_OK;
-            }
-        }
-        // group sizes of the element types the solve sweeps with
-        std::vector<int> sizes;
-        if (sweeps_f64()) { sizes.push_back(column_group<double>()); }
-        else if (fixed32()) { sizes.push_back(column_group<uint32_t>()); }
-        else { sizes.push_back(column_group<float>()); }
-        if (polish() && sizes[0] != column_group<double>())
-        {
-            sizes.push_back(column_group<double>());
-        }
-        for (int c = 0; c < nChains; ++c)
-        {
-            for (int G : sizes)
-            {
-                if (chainBegin[c + 1] - chainBegin[c] < G) { return EKX_OK; }
-            }
-        }
-        const int nbx = (g0.nx + kColumnTile - 1)/kColumnTile;
-        const int nby = (g0.ny + kColumnTile - 1)/kColumnTile;
-        const int nbz = (g0.nz + chunkH - 1)/chunkH;
-        for (size_t k = 0; k < sizes.size(); ++k)
-        {
-            GroupItems &gi = groupItems[k];
-            gi.G = sizes[k];
-            std::vector<ColumnItem> flat;
-            for (int c = 0; c < nChains; ++c)
-            {
-                std::vector<std::vector<ColumnItem>> perDiag(nDiag);
-                for (int first = chainBegin[c]; first < chainBegin[c + 1];
-                     first += gi.G)
-                {
-                    for (int bz = 0; bz < nbz; ++bz)
-                    {
-                        for (int by = 0; by < nby; ++by)
-                        {
-                            for (int bx = 0; bx < nbx; ++bx)
-                            {
-                                perDiag[bx + by + bz].push_back(
-                                    ColumnItem{first, bx, by, bz});
-                            }
-                        }
-                    }
-                }
-                for (auto &d : perDiag)
-                {
-                    gi.diagStart.push_back((int) flat.size());
-                    flat.insert(flat.end(), d.begin(), d.end());
-                }
-                gi.diagStart.push_back((int) flat.size());
-            }
-            EKX_CHECK(gi.items.allocate(flat.size()));
-            EKX_HIP(hipMemcpyAsync(gi.items.ptr, flat.data(),
-                                   flat.size()*sizeof(ColumnItem),
-                                   hipMemcpyHostToDevice, stream));
-        }
-        grouped = true;
-        return EKX_OK;
-    }
-
-    // The group work list for element type TS, or NULL.
-    template <typename TS> const GroupItems *group_items() const
-    {
-        if (!grouped) { return nullptr; }
-        for (const GroupItems &g : groupItems)
-        {
-            if (g.G == column_group<TS>()) { return &g; }
-        }
-        return nullptr;
     }
 
     // Uploads one model (cell slowness, double) to its slot.
@@ -955,6 +778,38 @@ struct Engine3D
                 EKX_HIP(launch_encode_slowness(modelCells[m], sm, 1.0/step,
                                                SQ.ptr + modelOffset[m],
                                                stream));
+            }
+        }
+        if (tiled)
+        {
+            // slowness bricks of every model, in each element type swept
+            for (size_t m = 0; m < modelCells.size(); ++m)
+            {
+                const ekx_geometry3d *g = nullptr;
+                for (size_t i = 0; i < problems.size() && !g; ++i)
+                {
+                    if (modelOf[i] == (int) m) { g = &geometry[i]; }
+                }
+                if (!g) { continue; }
+                const int64_t so = modelOffset[m], bo = modelBrickOffset[m];
+                if (SB64.ptr)
+                {
+                    EKX_HIP(launch_tile_slowness<double>(
+                        g->nx, g->ny, g->nz, S64.ptr + so, SB64.ptr + bo,
+                        stream));
+                }
+                if (SB32.ptr)
+                {
+                    EKX_HIP(launch_tile_slowness<float>(
+                        g->nx, g->ny, g->nz, S32.ptr + so, SB32.ptr + bo,
+                        stream));
+                }
+                if (SBQ.ptr)
+                {
+                    EKX_HIP(launch_tile_slowness<uint32_t>(
+                        g->nx, g->ny, g->nz, SQ.ptr + so, SBQ.ptr + bo,
+                        stream));
+                }
             }
         }
         EKX_HIP(hipStreamSynchronize(stream));
@@ -1025,19 +880,14 @@ struct Engine3D
     }
 
     // The factored-box columns need their own launch unless the bulk
-    // instance runs them (FP64, one problem per wave).
-    template <typename TS> static bool separate_factored(const void *group)
+    // instance runs them (FP64).
+    template <typename TS> static constexpr bool separate_factored()
     {
-        return !merged_factored<TS>() || group != nullptr;
+        return !merged_factored<TS>();
     }
 
-    // kernel_timing times the launches of the dominant bulk kernel: the
-    // grouped refinement launches when the solve groups, else every bulk
-    // column launch.
-    template <typename TS> bool timed_sweep(bool initialize) const
-    {
-        return timing && (initialize ? group_items<TS>() == nullptr : true);
-    }
+    // kernel_timing times every bulk column launch (the dominant kernel).
+    template <typename TS> bool timed_sweep(bool) const { return timing; }
 
     // Work counters of one sweep over every problem.
     template <typename TS>
@@ -1057,10 +907,8 @@ struct Engine3D
                 st.updates_factored += box;
                 if (timed_sweep<TS>(initialize))
                 {
-                    const bool merged =
-                        !separate_factored<TS>(initialize ? nullptr
-                                                          : group_items<TS>());
-                    st.timed_updates += merged ? upd : upd - box;
+                    st.timed_updates += separate_factored<TS>() ? upd - box
+                                                                : upd;
                 }
             }
         }
@@ -1079,76 +927,52 @@ struct Engine3D
         return rc;
     }
 
+    // Slowness bricks of element type TS (the column kernel's model).
+    template <typename TS> const TS *bricks() const
+    {
+        if constexpr (sizeof(TS) == 8) { return SB64.ptr; }
+        else if constexpr (std::is_same<TS, float>::value) { return SB32.ptr; }
+        else { return SBQ.ptr; }
+    }
+
     // One sweep of chain c on its own stream: per diagonal, the bulk
     // launch over the chain's items, then the factored-box columns of the
     // chain's problems.
-    // With the sweep overlap, odd sweeps of a list run on the chain's
-    // second stream: `parity` selects stream and diagonal events, and
-    // waitPrevious makes each diagonal wait for the previous sweep's
-    // diagonals it depends on.
     template <typename TS>
     ekx_status enqueue_chain_sweep(int c, int sweep, bool initialize, TS *T,
-                                   const TS *S, int64_t &launches,
-                                   int parity = 0, bool waitPrevious = false)
+                                   int64_t &launches)
     {
-        hipStream_t cs = parity ? chainStreams2[c] : chainStreams[c];
-        const bool ov = overlap();
-        hipEvent_t *evThis = ov ? diagEvents.data() +
-                                      ((size_t) c*2 + parity)*nDiag
-                                : nullptr;
-        hipEvent_t *evPrev = ov ? diagEvents.data() +
-                                      ((size_t) c*2 + (1 - parity))*nDiag
-                                : nullptr;
-        const int *req = ov ? reqDiag.data() + ((size_t) c*8 + sweep)*nDiag
-                            : nullptr;
-        int waited = -1;
+        hipStream_t cs = chainStreams[c];
+        const TS *SB = bricks<TS>();
         const int p0 = chainBegin[c];
         const int np = chainBegin[c + 1] - p0;
-        const GroupItems *gi = initialize ? nullptr : group_items<TS>();
-        const int *ds = (gi ? gi->diagStart.data() : diagStart.data()) +
-                        (size_t) c*(nDiag + 1);
+        const int *ds = diagStart.data() + (size_t) c*(nDiag + 1);
         for (int d = 0; d < nDiag; ++d)
         {
             const int count = ds[d + 1] - ds[d];
-            if (waitPrevious && req[d] > waited)
-            {
-                waited = req[d];
-                EKX_HIP(hipStreamWaitEvent(cs, evPrev[waited], 0));
-            }
             if (count > 0)
             {
                 if (timed_sweep<TS>(initialize))
                 {
                     EKX_CHECK(record_timing_event(cs));
                 }
-                if (gi)
-                {
-                    EKX_HIP(launch_group_columns3d<TS>(
-                        dProblems.ptr, gi->items.ptr + ds[d], count, sweep,
-                        d, boxPerDiag, chunkH, p0 + np, T, S, change.ptr,
-                        cs));
-                }
-                else
-                {
-                    EKX_HIP(launch_columns3d<TS>(
-                        dProblems.ptr, dItems.ptr + ds[d], count, sweep,
-                        initialize ? 1 : 0, d, boxPerDiag, chunkH, T, S,
-                        change.ptr, depth(), cs));
-                }
+                EKX_HIP(launch_columns3d<TS>(
+                    dProblems.ptr, dItems.ptr + ds[d], count, sweep,
+                    initialize ? 1 : 0, d, boxPerDiag, chunkH, T, SB,
+                    change.ptr, depth(), cs));
                 if (timed_sweep<TS>(initialize))
                 {
                     EKX_CHECK(record_timing_event(cs));
                 }
                 launches += 1;
             }
-            if (boxPerDiag > 0 && separate_factored<TS>(gi))
+            if (boxPerDiag > 0 && separate_factored<TS>())
             {
                 EKX_HIP(launch_factored_columns3d<TS>(
                     dProblems.ptr + p0, np, sweep, initialize ? 1 : 0, d,
-                    boxPerDiag, chunkH, T, S, change.ptr, cs));
+                    boxPerDiag, chunkH, T, SB, change.ptr, cs));
                 launches += 1;
             }
-            if (ov) { EKX_HIP(hipEventRecord(evThis[d], cs)); }
         }
         return EKX_OK;
     }
@@ -1183,38 +1007,29 @@ struct Engine3D
             // Per diagonal: the factored-box columns run on the side stream
             // concurrently with the bulk (Cartesian) launch; both join
             // before the next diagonal.
-            const GroupItems *gi = initialize ? nullptr : group_items<TS>();
-            const std::vector<int> &ds = gi ? gi->diagStart : diagStart;
+            const TS *SB = bricks<TS>();
+            const std::vector<int> &ds = diagStart;
             for (int d = 0; d < nDiag; ++d)
             {
                 int count = ds[d + 1] - ds[d];
-                const bool side = boxPerDiag > 0 && separate_factored<TS>(gi);
+                const bool side = boxPerDiag > 0 && separate_factored<TS>();
                 if (side)
                 {
                     EKX_HIP(hipEventRecord(evFork, stream));
                     EKX_HIP(hipStreamWaitEvent(sideStream, evFork, 0));
                     EKX_HIP(launch_factored_columns3d<TS>(
                         dProblems.ptr, n, sweep, initialize ? 1 : 0, d,
-                        boxPerDiag, chunkH, T, S, change.ptr, sideStream));
+                        boxPerDiag, chunkH, T, SB, change.ptr, sideStream));
                     launches += 1;
                 }
                 if (timed_sweep<TS>(initialize))
                 {
                     EKX_CHECK(record_timing_event(stream));
                 }
-                if (gi)
-                {
-                    EKX_HIP(launch_group_columns3d<TS>(
-                        dProblems.ptr, gi->items.ptr + ds[d], count, sweep,
-                        d, boxPerDiag, chunkH, n, T, S, change.ptr, stream));
-                }
-                else
-                {
-                    EKX_HIP(launch_columns3d<TS>(
-                        dProblems.ptr, dItems.ptr + ds[d], count, sweep,
-                        initialize ? 1 : 0, d, boxPerDiag, chunkH, T, S,
-                        change.ptr, depth(), stream));
-                }
+                EKX_HIP(launch_columns3d<TS>(
+                    dProblems.ptr, dItems.ptr + ds[d], count, sweep,
+                    initialize ? 1 : 0, d, boxPerDiag, chunkH, T, SB,
+                    change.ptr, depth(), stream));
                 if (timed_sweep<TS>(initialize))
                 {
                     EKX_CHECK(record_timing_event(stream));
@@ -1269,12 +1084,7 @@ struct Engine3D
         {
             for (int s = 0; s < 8; ++s) { list.push_back({s, false}); }
         }
-        if (iterations > 0 && kernel() == EKX_KERNEL_COLUMN &&
-            group_items<TS>() != nullptr)
-        {
-            st.group_waves = column_group<TS>();
-        }
-        if (nChains == 1 && !overlap())
+        if (nChains == 1)
         {
             for (const auto &sw : list)
             {
@@ -1284,9 +1094,7 @@ struct Engine3D
         else
         {
             // Chains run their whole sweep list independently: fork from
-            // the main stream, join back at the end.  With the overlap,
-            // sweep i of the list runs on the chain's stream i mod 2.
-            const bool ov = overlap();
+            // the main stream, join back at the end.
             for (const auto &sw : list)
             {
                 count_sweep<TS>(sw.first, sw.second, st);
@@ -1296,31 +1104,16 @@ struct Engine3D
             for (int c = 0; c < nChains; ++c)
             {
                 EKX_HIP(hipStreamWaitEvent(chainStreams[c], evFork, 0));
-                if (ov)
+                for (const auto &sw : list)
                 {
-                    EKX_HIP(hipStreamWaitEvent(chainStreams2[c], evFork, 0));
-                }
-                for (size_t i = 0; i < list.size(); ++i)
-                {
-                    const int parity = ov ? (int) (i & 1) : 0;
-                    EKX_CHECK(enqueue_chain_sweep<TS>(
-                        c, list[i].first, list[i].second, T, S, launches,
-                        parity, ov && i > 0));
+                    EKX_CHECK(enqueue_chain_sweep<TS>(c, sw.first, sw.second,
+                                                      T, launches));
                 }
                 EKX_HIP(hipEventRecord(chainEvents[c], chainStreams[c]));
-                if (ov)
-                {
-                    EKX_HIP(hipEventRecord(chainEvents2[c],
-                                           chainStreams2[c]));
-                }
             }
             for (int c = 0; c < nChains; ++c)
             {
                 EKX_HIP(hipStreamWaitEvent(stream, chainEvents[c], 0));
-                if (ov)
-                {
-                    EKX_HIP(hipStreamWaitEvent(stream, chainEvents2[c], 0));
-                }
             }
             st.kernel_launches += launches;
             if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
@@ -1378,7 +1171,6 @@ struct Engine3D
         st.kernel_launches_fp64 += local.kernel_launches_fp64;
         st.iterations_run += local.iterations_run;
         st.timed_updates += local.timed_updates;
-        st.group_waves = std::max(st.group_waves, local.group_waves);
         return EKX_OK;
     }
 
@@ -1444,7 +1236,7 @@ struct Engine3D
         EKX_HIP(hipEventRecord(evStart, stream));
         if (sweeps_f64())
         {
-            EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, maxNodes,
+            EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, maxCount,
                                           T64.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<double>(T64.ptr, S64.ptr, nIt, st));
@@ -1452,7 +1244,7 @@ struct Engine3D
         }
         else if (fixed32())
         {
-            EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, maxNodes,
+            EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, maxCount,
                                             TQ.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<uint32_t>(TQ.ptr, SQ.ptr, itPrimary, st));
@@ -1460,7 +1252,7 @@ struct Engine3D
         }
         else
         {
-            EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, maxNodes,
+            EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, maxCount,
                                          T32.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<float>(T32.ptr, S32.ptr, itPrimary, st));
@@ -1470,7 +1262,7 @@ struct Engine3D
         {
             if (fixed32())
             {
-                EKX_HIP(launch_decode_times(dProblems.ptr, nProb, maxNodes,
+                EKX_HIP(launch_decode_times(dProblems.ptr, nProb, maxCount,
                                             TQ.ptr, T64.ptr, stream));
             }
             else
@@ -1510,6 +1302,7 @@ struct Engine3D
             st.timed_launches = (int64_t) (timingUsed/2);
             // timed_updates: counted per timed sweep by count_sweep
         }
+        st.tiled = tiled ? 1 : 0;
         stats = st;
         haveField = true;
         solveStamp += 1;
@@ -1665,6 +1458,41 @@ struct Engine3D
         return EKX_OK;
     }
 
+    // Device pointer to problem's field in natural order, in its stored
+    // type: the field itself, or (tiled) a copy kept until the next solve.
+    template <typename TS>
+    ekx_status natural_field(int problem, const TS *T, const TS **out) const
+    {
+        const Problem3D &p = problems[problem];
+        if (!tiled)
+        {
+            *out = T + p.tOffset;
+            return EKX_OK;
+        }
+        DeviceBuffer<TS> *buf;
+        if constexpr (sizeof(TS) == 8) { buf = &natural64; }
+        else if constexpr (std::is_same<TS, float>::value)
+        {
+            buf = &natural32;
+        }
+        else { buf = &naturalQ; }
+        if (naturalProblem != problem || naturalStamp != solveStamp ||
+            !buf->ptr)
+        {
+            naturalProblem = -1;
+            natural64.release();
+            natural32.release();
+            naturalQ.release();
+            EKX_CHECK(buf->allocate((size_t) maxNodes));
+            EKX_HIP(launch_untile_field<TS>(p, T, buf->ptr, stream));
+            EKX_HIP(hipStreamSynchronize(stream));
+            naturalProblem = problem;
+            naturalStamp = solveStamp;
+        }
+        *out = buf->ptr;
+        return EKX_OK;
+    }
+
     ekx_status copy_field(int problem, int64_t n, double *out) const
     {
         const ekx_geometry3d &g = geometry[problem];
@@ -1676,18 +1504,20 @@ struct Engine3D
                                         "points " + std::to_string(nGrid));
         }
         if (!out) { return fail(EKX_EINVAL, "output is NULL"); }
-        const int64_t off = problems[problem].tOffset;
         if (fieldIsF64)
         {
-            EKX_HIP(hipMemcpy(out, T64.ptr + off, (size_t) n*sizeof(double),
+            const double *src = nullptr;
+            EKX_CHECK(natural_field<double>(problem, T64.ptr, &src));
+            EKX_HIP(hipMemcpy(out, src, (size_t) n*sizeof(double),
                               hipMemcpyDeviceToHost));
         }
         else if (TQ.ptr)
         {
             // the decoding of Store<uint32_t>::time
+            const uint32_t *src = nullptr;
+            EKX_CHECK(natural_field<uint32_t>(problem, TQ.ptr, &src));
             std::vector<uint32_t> tmp((size_t) n);
-            EKX_HIP(hipMemcpy(tmp.data(), TQ.ptr + off,
-                              (size_t) n*sizeof(uint32_t),
+            EKX_HIP(hipMemcpy(tmp.data(), src, (size_t) n*sizeof(uint32_t),
                               hipMemcpyDeviceToHost));
             const double step = problems[problem].tStep;
             for (int64_t i = 0; i < n; ++i)
@@ -1698,9 +1528,10 @@ struct Engine3D
         }
         else
         {
+            const float *src = nullptr;
+            EKX_CHECK(natural_field<float>(problem, T32.ptr, &src));
             std::vector<float> tmp((size_t) n);
-            EKX_HIP(hipMemcpy(tmp.data(), T32.ptr + off,
-                              (size_t) n*sizeof(float),
+            EKX_HIP(hipMemcpy(tmp.data(), src, (size_t) n*sizeof(float),
                               hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < n; ++i) { out[i] = tmp[(size_t) i]; }
         }
@@ -1849,8 +1680,6 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->chains = 0;
     o->prefetch_planes = 0;
     o->chunk_planes = 0;
-    o->column_group = 0;
-    o->sweep_overlap = 0;
 }
 
 ekx_status ekx_device_count(int *count)
@@ -2134,7 +1963,13 @@ ekx_status ekx_solver3d_get_travel_times_f32(const ekx_solver3d *h,
     }
     if (e.T32.ptr && !e.fieldIsF64)
     {
-        EKX_HIP(hipMemcpy(out, e.T32.ptr, (size_t) n*sizeof(float),
+        if (!e.haveField)
+        {
+            return fail(EKX_ESTATE, "Travel time field not yet computed");
+        }
+        const float *src = nullptr;
+        EKX_CHECK(e.natural_field<float>(0, e.T32.ptr, &src));
+        EKX_HIP(hipMemcpy(out, src, (size_t) n*sizeof(float),
                           hipMemcpyDeviceToHost));
         return EKX_OK;
     }
@@ -2162,8 +1997,18 @@ ekx_status ekx_solver3d_get_travel_times_device(const ekx_solver3d *h,
     }
     if (ptr)
     {
-        *ptr = e.fieldIsF64 ? (const void *) e.T64.ptr
-                            : (const void *) e.T32.ptr;
+        if (e.fieldIsF64)
+        {
+            const double *src = nullptr;
+            EKX_CHECK(e.natural_field<double>(0, e.T64.ptr, &src));
+            *ptr = src;
+        }
+        else
+        {
+            const float *src = nullptr;
+            EKX_CHECK(e.natural_field<float>(0, e.T32.ptr, &src));
+            *ptr = src;
+        }
     }
     if (element_size) { *element_size = e.fieldIsF64 ? 8 : 4; }
     return EKX_OK;

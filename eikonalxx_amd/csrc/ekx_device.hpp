@@ -22,6 +22,36 @@ constexpr int kColumnSkew = 2*(kColumnTile - 1);      // 14: lane a+b spread
 // are enough.
 constexpr int kColumnRing = 16;
 
+// Tiled field layout of the column kernel.  The nodes of an (x, y) tile of
+// 8 x 8 form one run of nz plane records of 64 values (x fastest within a
+// record); tiles follow in row-major (tile y, tile x) order and the grid is
+// padded to whole tiles.  A column therefore reads and writes one contiguous
+// run per sweep, in either z direction.  After a problem's tile records come
+// its edge records, one per (tile, plane), of kEdgeValues values: the
+// column's most downwind x column of its 9 x 9 working plane (the halo row
+// value first, then the 8 tile rows) and its most downwind y row (the 8
+// tile columns), as the current sweep leaves them for the next column in x
+// and in y.  Slowness bricks hold, per (tile, cell plane), the 9 x 9 cells
+// around the tile's nodes (cells 8 t - 1 .. 8 t + 7 per axis, clamped to the
+// grid), so a column's cells are one contiguous run too.
+constexpr int kTileValues = kColumnTile*kColumnTile;    // 64
+constexpr int kEdgeValues = 2*kColumnTile + 1;           // 17
+constexpr int kBrickValues = kColumnPlane;               // 81
+
+__host__ __device__ inline int tiles_of(int n)
+{
+    return (n + kColumnTile - 1)/kColumnTile;
+}
+
+// Index of node (x, y, z) in a tiled field of nTX tiles per row.
+__host__ __device__ inline int64_t tiled_index(int nTX, int nz, int x, int y,
+                                               int z)
+{
+    const int64_t tile = (int64_t) (y/kColumnTile)*nTX + x/kColumnTile;
+    return (tile*nz + z)*kTileValues + (y % kColumnTile)*kColumnTile +
+           x % kColumnTile;
+}
+
 // Grid constants of the non-uniform local solver
 // (include/private/solver3d.hpp:247-261).
 struct NonUniform3D
@@ -48,7 +78,26 @@ struct Problem3D
     // slowness = code * sStep; tScale = 1 / tStep.  Unused otherwise.
     double tStep, tScale, sStep;
     NonUniform3D nu;
+    // Field layout: tiled (column kernel) or natural (x fastest).  A tiled
+    // field is nTX * nTY * nz tile records, then as many edge records
+    // starting at eOffset; tCount is the field's whole extent (elements
+    // from tOffset).  sbOffset is the first element of the model's
+    // slowness bricks.
+    int tiled;
+    int nTX, nTY;
+    int pad;
+    int64_t tCount;
+    int64_t eOffset;
+    int64_t sbOffset;
 };
+
+// Index of node (x, y, z) in problem p's field (relative to tOffset).
+__host__ __device__ inline int64_t node_index(const Problem3D &p, int x, int y,
+                                              int z)
+{
+    return p.tiled ? tiled_index(p.nTX, p.nz, x, y, z)
+                   : ((int64_t) z*p.ny + y)*p.nx + x;
+}
 
 // A column chunk (8 x 8 footprint, one z-chunk) of one problem.
 struct ColumnItem

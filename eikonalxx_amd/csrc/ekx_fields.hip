@@ -16,33 +16,32 @@ namespace ekx
 namespace
 {
 
+// Travel time of node (x, y, z) of problem p's field (natural or tiled).
 template <typename TS>
-__device__ __forceinline__ double field_at(const TS *Tp, int64_t i,
-                                           const Problem3D &p)
+__device__ __forceinline__ double field_at(const TS *Tp, const Problem3D &p,
+                                           int x, int y, int z)
 {
-    return (double) Store<TS>::time(Tp[i], p);
+    return (double) Store<TS>::time(Tp[node_index(p, x, y, z)], p);
 }
 
-// Second-order derivative along one axis of stride s at index position
-// idx of n: central inside, the one-sided 3-point formulas on the edges
-// (src/gradient.hpp:100-181).
+// Second-order derivative along axis `axis` at node (x, y, z), whose index
+// on that axis is idx of n: central inside, the one-sided 3-point formulas
+// on the edges (src/gradient.hpp:100-181).
 template <typename TS>
-__device__ __forceinline__ double axis_derivative(const TS *Tp, int64_t i,
-                                                  int64_t s, int idx, int n,
-                                                  double u, double twoInv,
-                                                  const Problem3D &p)
+__device__ __forceinline__ double axis_derivative(const TS *Tp,
+                                                  const Problem3D &p, int x,
+                                                  int y, int z, int axis,
+                                                  int idx, int n, double u,
+                                                  double twoInv)
 {
-    if (idx == 0)
+    const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+    auto at = [&](int k)
     {
-        return (-3*u + 4*field_at(Tp, i + s, p) - field_at(Tp, i + 2*s, p))*
-               twoInv;
-    }
-    if (idx == n - 1)
-    {
-        return (field_at(Tp, i - 2*s, p) - 4*field_at(Tp, i - s, p) + 3*u)*
-               twoInv;
-    }
-    return (field_at(Tp, i + s, p) - field_at(Tp, i - s, p))*twoInv;
+        return field_at(Tp, p, x + k*ex, y + k*ey, z + k*ez);
+    };
+    if (idx == 0) { return (-3*u + 4*at(1) - at(2))*twoInv; }
+    if (idx == n - 1) { return (at(-2) - 4*at(-1) + 3*u)*twoInv; }
+    return (at(1) - at(-1))*twoInv;
 }
 
 } // namespace
@@ -76,9 +75,7 @@ stations3d_kernel(const Problem3D *__restrict__ problems,
         return;
     }
     const TS *Tp = T + p.tOffset;
-    const int64_t sy = p.nx, sz = (int64_t) p.nx*p.ny;
-    const int64_t i000 = ((int64_t) l.cell[2]*p.ny + l.cell[1])*p.nx +
-                         l.cell[0];
+    const int cx = l.cell[0], cy = l.cell[1], cz = l.cell[2];
     const double dxi = 1./p.dx, dyi = 1./p.dy, dzi = 1./p.dz;
     const double x0 = l.cell[0]*p.dx, x1 = (l.cell[0] + 1)*p.dx;
     const double y0 = l.cell[1]*p.dy, y1 = (l.cell[1] + 1)*p.dy;
@@ -90,11 +87,11 @@ stations3d_kernel(const Problem3D *__restrict__ problems,
 #pragma unroll
     for (int c = 0; c < 2; ++c)
     {
-        const int64_t b = i000 + c*sz;
-        const double fy0 = dxi*(x2mx*field_at(Tp, b, p) +
-                                xmx1*field_at(Tp, b + 1, p));
-        const double fy1 = dxi*(x2mx*field_at(Tp, b + sy, p) +
-                                xmx1*field_at(Tp, b + sy + 1, p));
+        const double fy0 = dxi*(x2mx*field_at(Tp, p, cx, cy, cz + c) +
+                                xmx1*field_at(Tp, p, cx + 1, cy, cz + c));
+        const double fy1 = dxi*(x2mx*field_at(Tp, p, cx, cy + 1, cz + c) +
+                                xmx1*field_at(Tp, p, cx + 1, cy + 1,
+                                              cz + c));
         f[c] = dyi*(y2my*fy0 + ymy1*fy1);
     }
     out[slot] = dzi*(z2mz*f[0] + zmz1*f[1]);
@@ -117,13 +114,12 @@ gradient3d_kernel(const Problem3D *__restrict__ problems, int k,
     const int iz = blockIdx.z;
     if (ix >= p.nx || iy >= p.ny) { return; }
     const TS *Tp = T + p.tOffset;
-    const int64_t sy = p.nx, sz = (int64_t) p.nx*p.ny;
-    const int64_t i = (int64_t) iz*sz + (int64_t) iy*sy + ix;
-    const double u = field_at(Tp, i, p);
+    const int64_t i = ((int64_t) iz*p.ny + iy)*p.nx + ix;   // output
+    const double u = field_at(Tp, p, ix, iy, iz);
     double g[3];
-    g[0] = axis_derivative(Tp, i, 1, ix, p.nx, u, 1./(2.*p.dx), p);
-    g[1] = axis_derivative(Tp, i, sy, iy, p.ny, u, 1./(2.*p.dy), p);
-    g[2] = axis_derivative(Tp, i, sz, iz, p.nz, u, 1./(2.*p.dz), p);
+    g[0] = axis_derivative(Tp, p, ix, iy, iz, 0, ix, p.nx, u, 1./(2.*p.dx));
+    g[1] = axis_derivative(Tp, p, ix, iy, iz, 1, iy, p.ny, u, 1./(2.*p.dy));
+    g[2] = axis_derivative(Tp, p, ix, iy, iz, 2, iz, p.nz, u, 1./(2.*p.dz));
     const int ox = ix - p.srcCellX, oy = iy - p.srcCellY;
     const int oz = iz - p.srcCellZ;
     if ((unsigned) ox <= 1u && (unsigned) oy <= 1u && (unsigned) oz <= 1u)

@@ -24,19 +24,11 @@ template <typename TS> constexpr bool merged_factored()
     return sizeof(TS) == 8;
 }
 
-// Waves (problems) per block of the grouped column kernel: one slowness
-// ring per block serves them all.  FP64: 6 travel-time rings of 11.4 KB and
-// the shared ring, 80 KB, two blocks (12 waves) per CU, within the 3 waves
-// per SIMD that its register count allows; 4-byte types: 8 rings of
-// 5.7 KB, three blocks per CU.
-template <typename TS> constexpr int column_group()
-{
-    return sizeof(TS) == 4 ? 8 : 6;
-}
-
+// HUGE everywhere in every problem's field (maxCount: the largest tCount),
+// the analytic times at the source cell's corners.
 template <typename TS>
 hipError_t launch_init3d(const Problem3D *dProblems, int nProblems,
-                         int64_t maxNodes, TS *T, hipStream_t stream);
+                         int64_t maxCount, TS *T, hipStream_t stream);
 template <typename TS>
 hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,
@@ -45,7 +37,8 @@ hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
 // One diagonal bx + by + bz of column chunks (8 x 8 columns cut into
 // z-chunks of `chunk` planes), Cartesian operators: the diagonal's items,
 // except the chunks overlapping a factored box (boxPerDiag > 0).  depth:
-// planes prefetched ahead of the compute (4, 8 or 12).
+// planes prefetched ahead of the compute (4, 8 or 12).  T: tiled fields
+// (with their edge records); S: slowness bricks.
 template <typename TS>
 hipError_t launch_columns3d(const Problem3D *dProblems,
                             const ColumnItem *dItems, int nItems, int sweep,
@@ -53,17 +46,6 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
                             int chunk, TS *T, const TS *S,
                             unsigned int *change, int depth,
                             hipStream_t stream);
-// Refinement sweeps (initialize = 0) of problems that share one model and
-// one geometry: items list the first problem of each group of
-// column_group<TS>() consecutive problems; problems from problemEnd on
-// are padding (no updates, no stores).
-template <typename TS>
-hipError_t launch_group_columns3d(const Problem3D *dProblems,
-                                  const ColumnItem *dItems, int nItems,
-                                  int sweep, int diagonal, int boxPerDiag,
-                                  int chunk, int problemEnd, TS *T,
-                                  const TS *S, unsigned int *change,
-                                  hipStream_t stream);
 // The same diagonal's chunks that overlap a factored box: up to
 // boxPerDiag per problem (the box's columns in x and y), full local
 // update.  Independent of the launch above (disjoint chunks), so the two
@@ -75,6 +57,15 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      TS *T, const TS *S,
                                      unsigned int *change,
                                      hipStream_t stream);
+// Tiled layout: the slowness bricks of one model (natural cells s of an
+// nx x ny x nz node grid) into b, and a natural-order copy (nx*ny*nz
+// values) of problem p's tiled field (T: the whole buffer).
+template <typename TS>
+hipError_t launch_tile_slowness(int nx, int ny, int nz, const TS *s, TS *b,
+                                hipStream_t stream);
+template <typename TS>
+hipError_t launch_untile_field(const Problem3D &p, const TS *T, TS *out,
+                               hipStream_t stream);
 // Sweep launches take `change`: NULL, or a device word that receives (as
 // float bits, by atomic max) the largest decrease any update of the launch
 // made -- the convergence monitor of the early-exit option.
@@ -93,7 +84,7 @@ hipError_t launch_block_max(int64_t n, const double *v, double *partial,
 hipError_t launch_encode_slowness(int64_t n, const double *in, double scale,
                                   uint32_t *out, hipStream_t stream);
 hipError_t launch_decode_times(const Problem3D *dProblems, int nProblems,
-                               int64_t maxNodes, const uint32_t *in,
+                               int64_t maxCount, const uint32_t *in,
                                double *out, hipStream_t stream);
 hipError_t launch_convert(int64_t n, const double *in, float *out,
                           hipStream_t stream);

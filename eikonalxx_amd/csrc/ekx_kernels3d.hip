@@ -36,10 +36,11 @@ __device__ __forceinline__ Octant sweep_octant(const Problem3D &p, int sweep,
     return o;
 }
 
-// Columns (8 x 8 footprints in the permuted frame) that overlap the
-// factored-eikonal box |i - iSrc| <= radius (solverUtilities3d.hpp:296-298),
-// in units of columns (x, y) and z-chunks of `chunk` planes.  Empty
-// (lo > hi) when the radius is negative.
+// Columns (8 x 8 tiles, indexed in the sweep's visiting order) that
+// overlap the factored-eikonal box |i - iSrc| <= radius
+// (solverUtilities3d.hpp:296-298), in units of columns (x, y) and z-chunks
+// of `chunk` planes of the permuted frame.  Empty (lo > hi) when the radius
+// is negative.
 struct BoxColumns
 {
     int bxLo, bxHi, byLo, byHi, bzLo, bzHi;
@@ -59,13 +60,16 @@ __device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
     int zLo = max(p.srcCellZ - p.radius, 0);
     int zHi = min(p.srcCellZ + p.radius, p.nz - 1);
     if (xLo > xHi || yLo > yHi || zLo > zHi) { return c; }
-    int jxLo = negX ? p.nx - 1 - xHi : xLo, jxHi = negX ? p.nx - 1 - xLo : xHi;
-    int jyLo = negY ? p.ny - 1 - yHi : yLo, jyHi = negY ? p.ny - 1 - yLo : yHi;
+    // tiles are aligned to the grid's own axes: tile t holds nodes
+    // 8 t .. 8 t + 7 in every sweep, and a sweep that reverses an axis
+    // visits its tiles in reverse order
+    const int mxLo = xLo/kColumnTile, mxHi = xHi/kColumnTile;
+    const int myLo = yLo/kColumnTile, myHi = yHi/kColumnTile;
+    c.bxLo = negX ? p.nTX - 1 - mxHi : mxLo;
+    c.bxHi = negX ? p.nTX - 1 - mxLo : mxHi;
+    c.byLo = negY ? p.nTY - 1 - myHi : myLo;
+    c.byHi = negY ? p.nTY - 1 - myLo : myHi;
     int jzLo = negZ ? p.nz - 1 - zHi : zLo, jzHi = negZ ? p.nz - 1 - zLo : zHi;
-    c.bxLo = jxLo/kColumnTile;
-    c.bxHi = jxHi/kColumnTile;
-    c.byLo = jyLo/kColumnTile;
-    c.byHi = jyHi/kColumnTile;
     c.bzLo = jzLo/chunk;
     c.bzHi = jzHi/chunk;
     return c;
@@ -74,7 +78,8 @@ __device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
 // ---------------------------------------------------------------------------
 // Source initialisation (Solver3DImpl::initializeTravelTimes,
 // src/solver3d.cpp:21-84): HUGE everywhere, the analytic time at the 8
-// corners of the source cell.  blockIdx.y is the problem.
+// corners of the source cell.  blockIdx.y is the problem.  Covers the whole
+// extent of a field (tile padding and edge records included).
 // ---------------------------------------------------------------------------
 template <typename TS>
 __global__ void __launch_bounds__(256)
@@ -83,15 +88,35 @@ init_travel_times3d(const Problem3D *__restrict__ problems,
 {
     const Problem3D &p = problems[blockIdx.y];
     const int64_t nxy = (int64_t) p.nx*p.ny;
-    const int64_t n = nxy*p.nz;
+    // tiled: tile records, then edge records (HUGE as well)
+    const int64_t nTileValues = (int64_t) p.nTX*p.nTY*p.nz*kTileValues;
     TS *Tp = T + p.tOffset;
-    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x*blockDim.x)
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+         i < p.tCount; i += (int64_t) gridDim.x*blockDim.x)
     {
-        int iz = (int) (i/nxy);
-        int rem = (int) (i - (int64_t) iz*nxy);
-        int iy = rem/p.nx;
-        int ix = rem - iy*p.nx;
+        int ix, iy, iz;
+        if (p.tiled)
+        {
+            if (i >= nTileValues)
+            {
+                Tp[i] = Store<TS>::kHugeCode;
+                continue;
+            }
+            const int64_t rec = i/kTileValues;
+            const int local = (int) (i - rec*kTileValues);
+            const int tile = (int) (rec/p.nz);
+            iz = (int) (rec - (int64_t) tile*p.nz);
+            const int ty = tile/p.nTX;
+            ix = (tile - ty*p.nTX)*kColumnTile + local % kColumnTile;
+            iy = ty*kColumnTile + local/kColumnTile;
+        }
+        else
+        {
+            iz = (int) (i/nxy);
+            int rem = (int) (i - (int64_t) iz*nxy);
+            iy = rem/p.nx;
+            ix = rem - iy*p.nx;
+        }
         int ox = ix - p.srcCellX, oy = iy - p.srcCellY, oz = iz - p.srcCellZ;
         TS v = Store<TS>::kHugeCode;
         if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1 && oz >= 0 && oz <= 1)
@@ -192,67 +217,61 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
 }
 
 // ---------------------------------------------------------------------------
-// Column kernel.  One 64-lane wave owns an 8 x 8 (jx, jy) column of one
-// problem and marches through z.  Lane (a, b) works on plane
-// q = step - (a + b) (relative to the first plane of the sweep), so at every
-// step the lanes hold nodes on one hyperplane and a lane's x- and
+// Column kernel.  One 64-lane wave owns an 8 x 8 (x, y) tile of one problem
+// and marches through z.  In the sweep's permuted frame lane (a, b) works on
+// plane q = step - (a + b) (relative to the first plane of the sweep), so at
+// every step the lanes hold nodes on one hyperplane and a lane's x- and
 // y-neighbours on the same plane were finished one or two steps earlier by
-// the neighbouring lanes.  Planes are staged in a 16-slot LDS ring of
-// 9 x 9 values (the 8 x 8 tile plus the upwind halo row and column, which
-// the previous column diagonal has already finished); the z-neighbours stay
-// in registers.  Planes are prefetched DL steps ahead and written back,
-// row-coalesced, once the last lane has finished them.  The waves of a
-// block are independent: there is no block barrier.
+// the neighbouring lanes.  Planes are staged in a 16-slot LDS ring of 9 x 9
+// values (the tile plus the upwind halo row and column, which the previous
+// column diagonal has already finished); the z-neighbours stay in
+// registers.  Planes are prefetched DL steps ahead and written back once the
+// last lane has finished them.  The waves of a block are independent: there
+// is no block barrier.
 //
-// Launch: one launch per column diagonal bx + by = d.  Columns on one
-// diagonal never read or write each other's nodes; their halos belong to
-// the diagonals before.
-// ---------------------------------------------------------------------------
+// Memory (tiled layout): the tile's plane records are one contiguous run,
+// read and written 64 values at a time; the halo comes from the edge
+// records that the upwind columns wrote in this sweep (their last x column
+// and y row), and the column writes its own edge records with each plane.
+// Where the upwind column did not run in this sweep (it lies outside an
+// initialisation sweep's octant) the halo is read from the tile records
+// directly: those nodes keep their values through the sweep.  Slowness
+// comes from the tile's bricks (9 x 9 cells per plane, contiguous).
+//
+// Launch: one launch per column diagonal bx + by (+ bz) = d, bx and by
+// counting tiles in the sweep's visiting order.  Columns on one diagonal
+// never read or write each other's nodes or edge records; their halos
+// belong to the diagonals before.
 //
 // Two instances per diagonal: FACTORED = false runs the Cartesian operators
-// only and skips the columns that overlap the factored box around the
-// source; FACTORED = true runs the full local update on exactly those
-// columns (at most boxPerDiag per problem and diagonal), with items
-// derived from the problem's source instead of the item list.  Keeping the
-// factored arithmetic out of the bulk kernel keeps its register count, and
-// so its occupancy, at that of the Cartesian update.
-//
-// GROUP > 1 (refinement sweeps of problems that share one model and one
-// geometry): the block's GROUP waves take the same column of GROUP
-// consecutive problems (item.problem is the first), and one slowness ring
-// serves them all, loaded cooperatively and published by a block barrier
-// every step.  Only the travel-time rings stay per wave, which nearly
-// halves the LDS per wave in FP64 (12 instead of 7 resident waves per CU).
-// Waves past problemEnd (the last group's padding) and waves whose column
-// overlaps their problem's factored box keep stepping (loads, barriers)
-// but update and store nothing.
+// (and, in FP64, the columns that overlap the factored box around the
+// source, with the full local update); FACTORED = true runs the full local
+// update on exactly the box columns (at most boxPerDiag per problem and
+// diagonal), with items derived from the problem's source instead of the
+// item list, for the 4-byte types, whose bulk instance keeps the register
+// count of the Cartesian update.
 // ---------------------------------------------------------------------------
-template <typename TS, int WAVES, int DL, bool FACTORED, int GROUP>
+template <typename TS, int WAVES, int DL, bool FACTORED>
 __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
                 const ColumnItem *__restrict__ items, int nItems, int sweep,
                 int initialize, int diagonal, int boxPerDiag, int chunk,
-                int problemEnd, TS *__restrict__ T, const TS *__restrict__ S,
+                TS *__restrict__ T, const TS *__restrict__ S,
                 unsigned int *__restrict__ change)
 {
-    constexpr bool kGroup = GROUP > 1;
-    static_assert(!kGroup || (WAVES == GROUP && !FACTORED),
-                  "a group is one block of GROUP waves, bulk instance");
     // bulk instance that also runs the factored-box columns
-    constexpr bool kMerged = !FACTORED && !kGroup && merged_factored<TS>();
+    constexpr bool kMerged = !FACTORED && merged_factored<TS>();
     bool boxColumn = false;
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
     // kColumnRing slots plus one spare that absorbs the stores of the
-    // padding steps after the last plane.  Travel times per wave; slowness
-    // per wave, or one ring for the whole group.
+    // padding steps after the last plane
     __shared__ TS ringT[WAVES][kColumnRing + 1][kSlot];
-    __shared__ TS ringS[kGroup ? 1 : WAVES][kColumnRing + 1][kSlot];
+    __shared__ TS ringS[WAVES][kColumnRing + 1][kSlot];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int itemIndex = kGroup ? (int) blockIdx.x : blockIdx.x*WAVES + wave;
-    if (itemIndex >= nItems) { return; }      // whole blocks in group mode
+    const int itemIndex = blockIdx.x*WAVES + wave;
+    if (itemIndex >= nItems) { return; }
     int pIndex, bx, by, bz;
-    bool active = true;
     if (FACTORED)
     {
         // item = (problem, k): the k-th (bx, by) of the box, at the z-chunk
@@ -274,12 +293,6 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         bx = item.bx;
         by = item.by;
         bz = item.bz;
-        if (kGroup)
-        {
-            pIndex += wave;
-            active = pIndex < problemEnd;
-            pIndex = min(pIndex, problemEnd - 1);
-        }
     }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
     const Problem3D &p = problems[pIndex];
@@ -291,20 +304,26 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         {
             // the full local update in this wave, or the FACTORED instance
             if (kMerged) { boxColumn = true; }
-            else if (!kGroup) { return; }
-            else { active = false; }
+            else { return; }
         }
     }
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const int ncx = nx - 1, ncy = ny - 1, ncz = nz - 1;
+    const int ncz = nz - 1;
+    const int nTX = p.nTX, nTY = p.nTY;
     const int negX = sweep & 1, negY = (sweep >> 1) & 1, negZ = (sweep >> 2) & 1;
     const Octant o = sweep_octant(p, sweep, initialize);
-    const int X0 = __builtin_amdgcn_readfirstlane(bx)*kColumnTile;
-    const int Y0 = __builtin_amdgcn_readfirstlane(by)*kColumnTile;
+    // The column's tile (mx, my) in grid order and its first node
+    // (X0, Y0) in the permuted frame; a reversed axis starts its tiles at
+    // the grid's far end, so X0 may be negative for the last tile.
+    const int mx = negX ? nTX - 1 - __builtin_amdgcn_readfirstlane(bx)
+                        : __builtin_amdgcn_readfirstlane(bx);
+    const int my = negY ? nTY - 1 - __builtin_amdgcn_readfirstlane(by)
+                        : __builtin_amdgcn_readfirstlane(by);
+    const int X0 = negX ? nx - kColumnTile*(mx + 1) : kColumnTile*mx;
+    const int Y0 = negY ? ny - kColumnTile*(my + 1) : kColumnTile*my;
     if (X0 + kColumnTile - 1 < o.jx0 || Y0 + kColumnTile - 1 < o.jy0)
     {
-        // column entirely outside the source octant (groups run
-        // refinement sweeps only, whose range is the same for the group)
+        // column entirely outside the source octant
         return;
     }
     // This item's planes: the chunk's [bz*chunk, (bz+1)*chunk) within the
@@ -317,98 +336,126 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int nPlanes = zEnd - P0;
     const int a = lane & 7, b = lane >> 3;
     const int jx = X0 + a, jy = Y0 + b;
-    const bool valid = jx < nx && jy < ny;
+    const bool valid = jx >= 0 && jx < nx && jy >= 0 && jy < ny;
     const bool upd = valid && jx >= o.jx0 && jy >= o.jy0;
     const int ix = negX ? nx - 1 - jx : jx;
     const int iy = negY ? ny - 1 - jy : jy;
     const Sweep3D sw = make_sweep3d(sweep);
-    TS *Tp = T + p.tOffset;
-    const TS *Sp = S + p.sOffset;
-    const int64_t nxy = (int64_t) nx*ny;
-    const int64_t ncxy = (int64_t) ncx*ncy;
 
-    // LDS plane layout: local coordinate (c, r), c = jx - X0 + 1,
-    // r = jy - Y0 + 1, both in 0..8, at r*9 + c (row-major 9 x 9, the
+    // LDS plane layout: local coordinate (c, r) of the permuted frame,
+    // c = jx - X0 + 1, r = jy - Y0 + 1, both in 0..8, at r*9 + c (the
     // upwind halo row and column at r = 0 and c = 0).  A lane's node is
     // (a+1, b+1): its x-1, y-1 and xy-1 neighbours and its 4 cells of a
     // plane sit at fixed offsets 9, 1, 0 and 10 from (a, b), so one address
     // per ring serves all reads.  Cells use the same layout with c, r the
     // cell's local indices.
     auto lidx = [](int c, int r) { return r*kColumnHalo + c; };
-    // Ring elements this lane loads: e0 = lane, e1 = lane + 64 (< 81).
-    // Every prefetch load is unconditional from a clamped, in-bounds
-    // address (lanes without a second element reload their first one);
-    // conditional loads would make the compiler wait for them at the merge
-    // point and serialise the prefetch.  Halo nodes outside the grid (the
-    // row or column before the first) hold their clamped neighbour's value:
-    // no update reads them, since updated nodes start at index 1 of every
-    // axis and read only index - 1.
-    const bool has1 = lane + 64 < kColumnPlane;
-    const int dst0 = lane;
-    const int dst1 = has1 ? lane + 64 : kColumnPlane; // padding if unused
-    // Group mode: thread t of the block loads slowness element t mod 81 of
-    // each plane into the shared ring (repeated elements are written with
-    // the same value), so every wave issues one slowness load per step.
-    const int sElem = kGroup ? (wave*64 + lane)%kColumnPlane : lane;
-    const int dstS = kGroup ? sElem : dst0;
-    int tXY0, tXY1, sXY0, sXY1;
+    // Tile record element e (grid-local x = e % 8, y = e / 8) sits at
+    // local (1 + x, 1 + y), mirrored along a reversed axis.
+    auto tileSlot = [&](int e)
     {
-        auto setup = [&](int e, int &tXY, int &sXY)
-        {
-            int c = e%kColumnHalo, r = e/kColumnHalo;
-            int gx = X0 - 1 + c, gy = Y0 - 1 + r;
-            int cgx = min(max(gx, 0), nx - 1);
-            int cgy = min(max(gy, 0), ny - 1);
-            int ax = negX ? nx - 1 - cgx : cgx;
-            int ay = negY ? ny - 1 - cgy : cgy;
-            tXY = ay*nx + ax;
-            int pcx = min(max(gx, 0), ncx - 1);
-            int pcy = min(max(gy, 0), ncy - 1);
-            int cx = negX ? ncx - 1 - pcx : pcx;
-            int cy = negY ? ncy - 1 - pcy : pcy;
-            sXY = cy*ncx + cx;
-        };
-        setup(lane, tXY0, sXY0);
-        setup(has1 ? lane + 64 : lane, tXY1, sXY1);
-        if (kGroup)
-        {
-            int unusedT;
-            setup(sElem, unusedT, sXY0);
-        }
-    }
+        const int xl = e % kColumnTile, yl = e/kColumnTile;
+        return lidx(1 + (negX ? kColumnTile - 1 - xl : xl),
+                    1 + (negY ? kColumnTile - 1 - yl : yl));
+    };
+    // Brick element e (cell 8 mx - 1 + e % 9 along x, likewise y) sits at
+    // local (e % 9, e / 9), mirrored along a reversed axis.
+    auto brickSlot = [&](int e)
+    {
+        const int cl = e % kColumnHalo, rl = e/kColumnHalo;
+        return lidx(negX ? kColumnTile - cl : cl, negY ? kColumnTile - rl : rl);
+    };
     using St = Store<TS>;
     using R = typename St::R;
-    // Plane addressing: wave-uniform plane pointers (scalar registers)
-    // plus per-lane byte offsets within a plane (32-bit), so the loads and
-    // the store take the scalar-base form and the per-step address work is
-    // a scalar pointer increment.
-    const uint32_t tB0 = (uint32_t) tXY0*sizeof(TS);
-    const uint32_t tB1 = (uint32_t) tXY1*sizeof(TS);
-    const uint32_t sB0 = (uint32_t) sXY0*sizeof(TS);
-    const uint32_t sB1 = (uint32_t) sXY1*sizeof(TS);
-    auto planeT = [&](int q) -> const char *
+    TS *Tp = T + p.tOffset;
+    const int64_t tileIndex = (int64_t) my*nTX + mx;
+    // Loads per lane and step: tile element `lane`, one halo value (lanes
+    // 0..8: the x-halo column c = 0, r = lane; lanes 9..16: the y-halo row
+    // r = 0, c = lane - 8; the other lanes repeat lane 16's load into the
+    // ring's padding), brick elements `lane` and `lane + 64` (lanes past
+    // the 81 cells repeat their first element into the padding).
+    // Every prefetch load is unconditional from an in-bounds address;
+    // conditional loads would make the compiler wait for them at the merge
+    // point and serialise the prefetch.
+    const int h = min(lane, kEdgeValues - 1);
+    const int dstOwn = tileSlot(lane);
+    const int dstHalo = lane < kColumnHalo ? lidx(0, lane)
+                      : (lane < kEdgeValues ? lidx(lane - kColumnTile, 0)
+                                            : kColumnPlane);
+    const bool has1 = lane + 64 < kColumnPlane;
+    const int dstS0 = brickSlot(lane);
+    const int dstS1 = has1 ? brickSlot(lane + 64) : kColumnPlane;
+    // Halo source: the upwind column's edge records (when that column ran
+    // in this sweep: its last node X0 - 1 is at or past the octant start),
+    // else the node itself in the tile records.  Per lane: the address at
+    // plane P0 and the per-plane stride in bytes.
+    const char *haloP0;
+    int32_t haloStep;
+    {
+        const int zP0 = negZ ? nz - 1 - P0 : P0;
+        const int dz = negZ ? -1 : 1;
+        const bool xHalo = h < kColumnHalo;
+        const bool fromEdge = xHalo ? X0 - 1 >= o.jx0 : Y0 - 1 >= o.jy0;
+        if (fromEdge)
+        {
+            const int ux = xHalo ? (negX ? mx + 1 : mx - 1) : mx;
+            const int uy = xHalo ? my : (negY ? my + 1 : my - 1);
+            const int64_t rec = ((int64_t) uy*nTX + ux)*nz + zP0;
+            haloP0 = (const char *) (T + p.eOffset + rec*kEdgeValues + h);
+            haloStep = dz*kEdgeValues*(int32_t) sizeof(TS);
+        }
+        else
+        {
+            // permuted (X0 - 1, Y0 - 1 + h) or (X0 - 1 + h - 8, Y0 - 1),
+            // clamped into the grid (clamped nodes are never read by an
+            // update: updated nodes start at index 1 and read index - 1)
+            const int hx = xHalo ? X0 - 1 : X0 - 1 + h - kColumnTile;
+            const int hy = xHalo ? Y0 - 1 + h : Y0 - 1;
+            const int cx = min(max(hx, 0), nx - 1);
+            const int cy = min(max(hy, 0), ny - 1);
+            const int gx = negX ? nx - 1 - cx : cx;
+            const int gy = negY ? ny - 1 - cy : cy;
+            haloP0 = (const char *) (Tp + tiled_index(nTX, nz, gx, gy, zP0));
+            haloStep = dz*kTileValues*(int32_t) sizeof(TS);
+        }
+    }
+    // Plane addressing: wave-uniform plane pointers (scalar registers) plus
+    // per-lane byte offsets, so the loads and the stores take the
+    // scalar-base form and the per-step address work is a scalar pointer
+    // increment (the halo address is per lane).
+    auto planeT = [&](int q) -> char *
     {
         const int jz = P0 + q;
         const int az = negZ ? nz - 1 - jz : jz;
-        return (const char *) (Tp + (int64_t) az*nxy);
+        return (char *) (Tp + (tileIndex*nz + az)*kTileValues);
+    };
+    auto planeE = [&](int q) -> char *
+    {
+        const int jz = P0 + q;
+        const int az = negZ ? nz - 1 - jz : jz;
+        return (char *) (T + p.eOffset + (tileIndex*nz + az)*kEdgeValues);
     };
     auto planeS = [&](int q) -> const char *
     {
         const int pcz = min(P0 + q, ncz - 1);
         const int cz = negZ ? ncz - 1 - pcz : pcz;
-        return (const char *) (Sp + (int64_t) cz*ncxy);
+        return (const char *) (S + p.sbOffset +
+                               (tileIndex*ncz + cz)*kBrickValues);
     };
+    const uint32_t tB = (uint32_t) lane*sizeof(TS);
+    const uint32_t sB0 = (uint32_t) lane*sizeof(TS);
+    const uint32_t sB1 = (uint32_t) (has1 ? lane + 64 : lane)*sizeof(TS);
     auto at = [](const char *base, uint32_t off) -> TS
     {
         return *(const TS *) (base + off);
     };
-    auto loadPlane = [&](const char *Tz, const char *Sz, TS &vT0, TS &vT1,
-                         TS &vS0, TS &vS1)
+    auto loadPlane = [&](const char *Tz, const char *Sz, const char *Hz,
+                         TS &vT0, TS &vT1, TS &vS0, TS &vS1)
     {
-        vT0 = at(Tz, tB0);
+        vT0 = at(Tz, tB);
         vS0 = at(Sz, sB0);
-        vT1 = at(Tz, tB1);
-        if (!kGroup) { vS1 = at(Sz, sB1); }
+        vT1 = *(const TS *) Hz;
+        vS1 = at(Sz, sB1);
     };
 
     TS bT0[DL], bT1[DL], bS0[DL], bS1[DL];
@@ -419,20 +466,27 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     for (int u = 0; u < DL; ++u)
     {
         const int q = min(u, nPlanes - 1);
-        loadPlane(planeT(q), planeS(q), bT0[u], bT1[u], bS0[u], bS1[u]);
+        loadPlane(planeT(q), planeS(q), haloP0 + (int64_t) q*haloStep,
+                  bT0[u], bT1[u], bS0[u], bS1[u]);
         __builtin_amdgcn_sched_barrier(0);
     }
     // Running plane pointers: the next prefetch (plane min(tc + DL,
     // nPlanes - 1)) and the next write-back (plane
     // clamp(tc - 15, 0, nPlanes - 1)); each advances by at most one plane
     // per step.
-    const int64_t stepT = (negZ ? -nxy : nxy)*(int64_t) sizeof(TS);
-    const int64_t stepS = (negZ ? -ncxy : ncxy)*(int64_t) sizeof(TS);
+    const int64_t stepT = (negZ ? -kTileValues : kTileValues)*
+                          (int64_t) sizeof(TS);
+    const int64_t stepE = (negZ ? -kEdgeValues : kEdgeValues)*
+                          (int64_t) sizeof(TS);
+    const int64_t stepS = (negZ ? -kBrickValues : kBrickValues)*
+                          (int64_t) sizeof(TS);
     int loadQ = min(DL, nPlanes - 1);
     const char *loadT = planeT(loadQ);
     const char *loadS = planeS(loadQ);
+    const char *loadH = haloP0 + (int64_t) loadQ*haloStep;
     int wbQ = 0;
-    char *wbT = (char *) planeT(0);
+    char *wbT = planeT(0);
+    char *wbE = planeE(0);
     // z-neighbours (plane q - 1) of the lane's node, decoded
     R pt4 = (R) kHuge, pt5 = (R) kHuge, pt6 = (R) kHuge, pt7 = (R) kHuge;
     R ps00 = 0, ps10 = 0, ps01 = 0, ps11 = 0;
@@ -440,19 +494,19 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // The node is local (a+1, b+1); oA is (a, b), the xy-1 neighbour.
     constexpr int kB = 1, kC = kColumnHalo, kD = kColumnHalo + 1;
     const int oA = lidx(a, b);
-    // write-back lane: itself, or the nearest in-grid lane of its row/column
-    const int wbA = min(a, nx - 1 - X0), wbB = min(b, ny - 1 - Y0);
-    const int wbOwn = lidx(wbA + 1, wbB + 1);
-    const int wbX = negX ? nx - 1 - (X0 + wbA) : X0 + wbA;
-    const int wbY = negY ? ny - 1 - (Y0 + wbB) : Y0 + wbB;
-    const uint32_t wbB8 = (uint32_t) (wbY*nx + wbX)*sizeof(TS);
+    // Write-back: tile element `lane`, and edge element h (lanes 0..8: the
+    // last column c = 8, r = h; lanes 9..16: the last row r = 8, c = h - 8;
+    // the other lanes repeat lane 16's store).
+    const int wbEdge = h < kColumnHalo ? lidx(kColumnTile, h)
+                                       : lidx(h - kColumnTile, kColumnTile);
+    const uint32_t eB = (uint32_t) h*sizeof(TS);
     // Steps: every plane passes every lane and is written back; rounded up
     // to whole groups of DL so the unrolled loop body has no exit (extra
-    // steps only repeat idempotent work: clamped loads, no computation, and
-    // a re-store of the last plane's final values).
+    // steps only repeat idempotent work: in-bounds loads, no computation,
+    // and a re-store of the last plane's final values).
     const int nSteps = ((nPlanes + kColumnSkew + 1 + DL - 1)/DL)*DL;
     TS(*myT)[kSlot] = ringT[wave];
-    TS(*myS)[kSlot] = ringS[kGroup ? 0 : wave];
+    TS(*myS)[kSlot] = ringS[wave];
     for (int t0 = 0; t0 < nSteps; t0 += DL)
     {
 #pragma unroll
@@ -460,41 +514,32 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         {
             const int tc = t0 + u;
             // (a) the plane prefetched DL steps ago enters the ring.  Branch
-            // free: past the last plane it goes to the spare slot, and lanes
-            // without a second element write the slot's padding.
+            // free: past the last plane it goes to the spare slot, and the
+            // repeated loads write the slot's padding.
             {
                 const int slot = tc < nPlanes ? (tc & (kColumnRing - 1))
                                               : kColumnRing;
                 TS *dT = myT[slot];
                 TS *dS = myS[slot];
-                dT[dst0] = bT0[u];
-                dS[dstS] = bS0[u];
-                dT[dst1] = bT1[u];
-                if (!kGroup) { dS[dst1] = bS1[u]; }
+                dT[dstOwn] = bT0[u];
+                dS[dstS0] = bS0[u];
+                dT[dstHalo] = bT1[u];
+                dS[dstS1] = bS1[u];
             }
             // (b) prefetch plane tc + DL into the freed registers (past the
             // last plane the load repeats it and is never stored)
-            loadPlane(loadT, loadS, bT0[u], bT1[u], bS0[u], bS1[u]);
+            loadPlane(loadT, loadS, loadH, bT0[u], bT1[u], bS0[u], bS1[u]);
             if (loadQ < nPlanes - 1)
             {
                 loadQ += 1;
                 loadT += stepT;
+                loadH += haloStep;
                 if (P0 + loadQ <= ncz - 1) { loadS += stepS; }
             }
             // (d, read) the plane the last lane finished a step ago; read
             // with the neighbour values below, stored after the update
-            const TS wbValue = myT[wbQ & (kColumnRing - 1)][wbOwn];
-            if (kGroup)
-            {
-                // the group's slowness plane entered above is read below by
-                // every wave: publish it (LDS only; global loads in flight
-                // stay in flight)
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup",
-                                       "local");
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup",
-                                       "local");
-            }
+            const TS wbValue = myT[wbQ & (kColumnRing - 1)][dstOwn];
+            const TS wbEdgeValue = myT[wbQ & (kColumnRing - 1)][wbEdge];
             // (c) every lane advances its node on plane tc - (a + b).  The
             // reads and the register shift run on every lane at every step
             // (outside the lane's planes they read unused ring slots and
@@ -514,7 +559,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 const R c10 = St::slowness(sS[kB], p);
                 const R c01 = St::slowness(sS[kC], p);
                 const R c11 = St::slowness(sS[kD], p);
-                if (upd && active && q > 0 && q < nPlanes)
+                if (upd && q > 0 && q < nPlanes)
                 {
                     const int jz = P0 + q;
                     const int iz = negZ ? nz - 1 - jz : jz;
@@ -555,16 +600,19 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 ps01 = c01;
                 ps11 = c11;
             }
-            // (d, store) the finished plane goes home.  The store is issued
-            // at every step (plane index clamped to the ring's valid range,
-            // unchanged nodes store their own value, lanes past the grid
-            // edge repeat an edge lane's store), so the prefetch loads above
-            // always have the same number of memory operations behind them.
-            if (active) { *(TS *) (wbT + wbB8) = wbValue; }
+            // (d, store) the finished plane goes home with its edge
+            // records.  The stores are issued at every step (plane index
+            // clamped to the ring's valid range; unchanged nodes and the
+            // tile's padding store their own values), so the prefetch loads
+            // above always have the same number of memory operations behind
+            // them.
+            *(TS *) (wbT + tB) = wbValue;
+            *(TS *) (wbE + eB) = wbEdgeValue;
             if (tc >= kColumnSkew + 1 && wbQ < nPlanes - 1)
             {
                 wbQ += 1;
                 wbT += stepT;
+                wbE += stepE;
             }
             // LDS traffic of this step must not be reordered with the next
             // step's: other lanes of the wave consume it.  The fence covers
@@ -588,6 +636,55 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         {
             atomicMax(change, __float_as_uint(d));
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled layout: slowness bricks and natural-order copies of fields.
+// ---------------------------------------------------------------------------
+// Brick element e of (tile (tx, ty), cell plane cz) = cell
+// (clamp(8 tx - 1 + e % 9), clamp(8 ty - 1 + e / 9), cz) of the natural
+// cell array.
+template <typename TS>
+__global__ void __launch_bounds__(256)
+tile_slowness_kernel(int nx, int ny, int nz, const TS *__restrict__ s,
+                     TS *__restrict__ b)
+{
+    const int ncx = nx - 1, ncy = ny - 1, ncz = nz - 1;
+    const int nTX = tiles_of(nx);
+    const int64_t n = (int64_t) nTX*tiles_of(ny)*ncz*kBrickValues;
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        const int64_t rec = i/kBrickValues;
+        const int e = (int) (i - rec*kBrickValues);
+        const int tile = (int) (rec/ncz);
+        const int cz = (int) (rec - (int64_t) tile*ncz);
+        const int ty = tile/nTX, tx = tile - ty*nTX;
+        const int cx = min(max(tx*kColumnTile - 1 + e % kColumnHalo, 0),
+                           ncx - 1);
+        const int cy = min(max(ty*kColumnTile - 1 + e/kColumnHalo, 0),
+                           ncy - 1);
+        b[i] = s[((int64_t) cz*ncy + cy)*ncx + cx];
+    }
+}
+
+template <typename TS>
+__global__ void __launch_bounds__(256)
+untile_field_kernel(Problem3D p, const TS *__restrict__ T,
+                    TS *__restrict__ out)
+{
+    const int64_t nxy = (int64_t) p.nx*p.ny;
+    const int64_t n = nxy*p.nz;
+    const TS *Tp = T + p.tOffset;
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        const int iz = (int) (i/nxy);
+        const int rem = (int) (i - (int64_t) iz*nxy);
+        const int iy = rem/p.nx;
+        const int ix = rem - iy*p.nx;
+        out[i] = Tp[tiled_index(p.nTX, p.nz, ix, iy, iz)];
     }
 }
 
@@ -718,7 +815,7 @@ decode_times_kernel(const Problem3D *__restrict__ problems,
                     const uint32_t *__restrict__ in, double *__restrict__ out)
 {
     const Problem3D &p = problems[blockIdx.y];
-    const int64_t n = (int64_t) p.nx*p.ny*p.nz;
+    const int64_t n = p.tCount;
     const uint32_t *src = in + p.tOffset;
     double *dst = out + p.tOffset;
     for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
@@ -776,43 +873,26 @@ hipError_t launch_columns3d(const Problem3D *dProblems,
     switch (depth)
     {
     case 4:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false, 1>), grid,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 4, false>), grid,
                            block, 0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
                            change);
         break;
     case 8:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false, 1>), grid,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 8, false>), grid,
                            block, 0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
                            change);
         break;
     case 12:
-        hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false, 1>), grid,
+        hipLaunchKernelGGL((sweep_columns3d<TS, W, 12, false>), grid,
                            block, 0, stream, dProblems, dItems, nItems, sweep,
-                           initialize, diagonal, boxPerDiag, chunk, 0, T, S,
+                           initialize, diagonal, boxPerDiag, chunk, T, S,
                            change);
         break;
     default:
         return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-template <typename TS>
-hipError_t launch_group_columns3d(const Problem3D *dProblems,
-                                  const ColumnItem *dItems, int nItems,
-                                  int sweep, int diagonal, int boxPerDiag,
-                                  int chunk, int problemEnd, TS *T,
-                                  const TS *S, unsigned int *change,
-                                  hipStream_t stream)
-{
-    if (nItems <= 0) { return hipSuccess; }
-    constexpr int G = column_group<TS>();
-    hipLaunchKernelGGL((sweep_columns3d<TS, G, 4, false, G>),
-                       dim3((unsigned) nItems), dim3(64*G), 0, stream,
-                       dProblems, dItems, nItems, sweep, 0, diagonal,
-                       boxPerDiag, chunk, problemEnd, T, S, change);
     return hipGetLastError();
 }
 
@@ -826,10 +906,33 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
 {
     if (boxPerDiag <= 0) { return hipSuccess; }
     int n = nProblems*boxPerDiag;
-    hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true, 1>),
+    hipLaunchKernelGGL((sweep_columns3d<TS, 1, 4, true>),
                        dim3((unsigned) n), dim3(64), 0, stream, dProblems,
                        (const ColumnItem *) nullptr, n, sweep, initialize,
-                       diagonal, boxPerDiag, chunk, 0, T, S, change);
+                       diagonal, boxPerDiag, chunk, T, S, change);
+    return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t launch_tile_slowness(int nx, int ny, int nz, const TS *s, TS *b,
+                                hipStream_t stream)
+{
+    const int64_t n = (int64_t) tiles_of(nx)*tiles_of(ny)*(nz - 1)*
+                      kBrickValues;
+    hipLaunchKernelGGL(tile_slowness_kernel<TS>,
+                       dim3(grid_for(n, 256, 16384)), dim3(256), 0, stream,
+                       nx, ny, nz, s, b);
+    return hipGetLastError();
+}
+
+template <typename TS>
+hipError_t launch_untile_field(const Problem3D &p, const TS *T, TS *out,
+                               hipStream_t stream)
+{
+    const int64_t n = (int64_t) p.nx*p.ny*p.nz;
+    hipLaunchKernelGGL(untile_field_kernel<TS>,
+                       dim3(grid_for(n, 256, 16384)), dim3(256), 0, stream,
+                       p, T, out);
     return hipGetLastError();
 }
 
@@ -920,29 +1023,30 @@ template hipError_t launch_hyperplane3d<uint32_t>(const Problem3D *, int,
 template hipError_t launch_columns3d<float>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     float *, const float *, unsigned int *, int, hipStream_t);
-template hipError_t launch_group_columns3d<float>(
-    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
-    float *, const float *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<float>(
     const Problem3D *, int, int, int, int, int, int, float *, const float *,
     unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<double>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     double *, const double *, unsigned int *, int, hipStream_t);
-template hipError_t launch_group_columns3d<double>(
-    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
-    double *, const double *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<double>(
     const Problem3D *, int, int, int, int, int, int, double *, const double *,
     unsigned int *, hipStream_t);
 template hipError_t launch_columns3d<uint32_t>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     uint32_t *, const uint32_t *, unsigned int *, int, hipStream_t);
-template hipError_t launch_group_columns3d<uint32_t>(
-    const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
-    uint32_t *, const uint32_t *, unsigned int *, hipStream_t);
 template hipError_t launch_factored_columns3d<uint32_t>(
     const Problem3D *, int, int, int, int, int, int, uint32_t *, const uint32_t *,
     unsigned int *, hipStream_t);
+#define EKX_TILED(TS)                                                        \
+    template hipError_t launch_tile_slowness<TS>(int, int, int, const TS *, \
+                                                 TS *, hipStream_t);       \
+    template hipError_t launch_untile_field<TS>(const Problem3D &,         \
+                                                const TS *, TS *,          \
+                                                hipStream_t);
+EKX_TILED(float)
+EKX_TILED(double)
+EKX_TILED(uint32_t)
+#undef EKX_TILED
 
 } // namespace ekx

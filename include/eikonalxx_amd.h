@@ -124,20 +124,6 @@ typedef struct
                                       bx+by+bz, shortening the dependency
                                       chain of a sweep); 0 (default):
                                       whole columns                        */
-    int column_group;              /* column kernel, refinement sweeps of a
-                                      batch whose problems share one model
-                                      and one geometry: 1 runs groups of
-                                      problems per block with one shared
-                                      slowness ring (more resident waves,
-                                      a block barrier per step); 0
-                                      (default): one problem per wave     */
-    int sweep_overlap;             /* column kernel: 1 lets each sweep start
-                                      its first column diagonals while the
-                                      previous sweep finishes its last ones
-                                      (every column waits only for the
-                                      columns of the previous sweep that
-                                      write or read its nodes or halo);
-                                      0 (default): sweep after sweep      */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */
@@ -170,9 +156,9 @@ typedef struct
                                      (kernel_timing)                       */
     int64_t timed_updates;        /* updates done by the timed launches    */
     int32_t iterations_run;       /* refinement iterations actually run    */
-    int32_t group_waves;          /* problems per block of the grouped
-                                     refinement launches (column_group);
-                                     0 when no launch was grouped          */
+    int32_t tiled;                /* 1: fields in the column kernel's tiled
+                                     layout (device copies in natural
+                                     order are made on request)           */
     double timed_kernel_ms;       /* summed durations of the timed launches */
     double last_iteration_change; /* early_exit: largest decrease of one
                                      update in the last refinement

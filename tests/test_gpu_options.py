@@ -174,46 +174,86 @@ def test_chunked_batch(ekx, oracle, chunk, chains):
         assert np.array_equal(b.get_travel_time_field(i), ref)
 
 
-@pytest.mark.parametrize("chains,chunk,precision,group", [
-    (1, 0, "FP64", 0), (4, 0, "FP64", 0), (3, 9, "FP64", 0),
-    (2, 0, "FP32", 0), (2, 0, "FIXED32_POLISH", 0), (2, 0, "FP64", 1)])
-def test_sweep_overlap_gives_the_same_fields(ekx, oracle, chains, chunk,
-                                             precision, group):
-    """Letting each sweep start while the previous one finishes only
-    reorders independent work: every field is bit-identical to the
-    sweep-after-sweep solve (and, in FP64, to the oracle)."""
+@pytest.mark.parametrize("chains,chunk,precision", [
+    (1, 0, "FP64"), (4, 0, "FP64"), (3, 9, "FP64"), (2, 0, "FP32"),
+    (2, 5, "FIXED32_POLISH"), (1, 7, "FIXED32")])
+def test_tiled_layout_matches_natural_layout(ekx, oracle, chains, chunk,
+                                             precision):
+    """The column kernel keeps fields in tiles of 8 x 8 nodes with edge
+    records; the hyperplane kernel keeps them in natural order.  Both run
+    the same arithmetic in a topological order, so an uneven batch (grids
+    that are not whole tiles, sources at corners and edges, chunked
+    columns, chains) gives bit-identical fields, station times and
+    gradients, and in FP64 the oracle's fields."""
     rng = np.random.default_rng(77 + chains + chunk)
-    nx, ny, nz, h = 37, 29, 26, 12.0
-    g = ekx.Geometry3D(nx, ny, nz, h)
-    slow = 1.0 / rng.uniform(1500.0, 5000.0,
-                             (nx - 1) * (ny - 1) * (nz - 1))
+    h = 12.0
     problems = []
-    for k in range(14):
-        src = tuple(float(rng.uniform(0, (n - 1) * h)) for n in (nx, ny, nz))
-        problems.append((g, slow, src))
-    # an uneven problem in the batch (different tiles) unless grouped
-    if not group:
-        g2 = ekx.Geometry3D(23, 31, 19, h)
-        s2 = 1.0 / rng.uniform(1500.0, 5000.0, 22 * 30 * 18)
-        problems.insert(3, (g2, s2, (100.0, 200.0, 50.0)))
+    for k, (nx, ny, nz) in enumerate([(37, 29, 26), (23, 31, 19),
+                                      (9, 17, 11), (16, 8, 5), (3, 3, 3)]):
+        g = ekx.Geometry3D(nx, ny, nz, h)
+        slow = 1.0 / rng.uniform(1500.0, 5000.0,
+                                 (nx - 1) * (ny - 1) * (nz - 1))
+        for src in [(0.0, 0.0, 0.0), ((nx - 1) * h, (ny - 1) * h,
+                                      (nz - 1) * h),
+                    tuple(float(rng.uniform(0, (n - 1) * h))
+                          for n in (nx, ny, nz))]:
+            problems.append((g, slow, src))
+    stations = [(5.0, 7.0, 3.0), (20.0, 11.0, 17.0)]   # inside every grid
 
-    def run(ov):
+    def run(kernel):
         b = ekx.Batch3D()
         b.initialize(ekx.SolverOptions(
             number_of_sweeps=3, factored_radius=3, chains=chains,
-            chunk_planes=chunk, column_group=group, sweep_overlap=ov,
+            chunk_planes=chunk, kernel=kernel,
             precision=getattr(ekx.Precision, precision)), problems)
+        b.set_stations(stations)
         b.solve()
         st = b.get_stats()
-        return [b.get_travel_time_field(i) for i in range(len(problems))], st
-    ref, st0 = run(0)
-    got, st1 = run(1)
-    assert st0.updates == st1.updates
-    for a, b in zip(ref, got):
+        fields = [b.get_travel_time_field(i) for i in range(len(problems))]
+        grads = [b.get_travel_time_gradient_field(i) for i in (0, 4, 9)]
+        return fields, b.get_station_table(), grads, st
+    tiled, tt, tg, st = run(ekx.Kernel.COLUMN)
+    assert st.tiled == 1
+    nat, nt, ng, st0 = run(ekx.Kernel.HYPERPLANE)
+    assert st0.tiled == 0
+    assert st0.updates == st.updates
+    for a, b in zip(tiled, nat):
         assert np.array_equal(a, b)
+    assert np.array_equal(tt, nt)
+    for a, b in zip(tg, ng):
+        assert np.array_equal(a, b, equal_nan=True)
     if precision == "FP64":
-        for i in (0, 3, len(problems) - 1):
+        for i in (0, 4, 8, len(problems) - 1):
             gg, slow_i, src = problems[i]
             og = oracle.geometry3d(gg.nx, gg.ny, gg.nz, h)
-            assert np.array_equal(got[i],
+            assert np.array_equal(tiled[i],
                                   oracle.solve3d(og, slow_i, src, 3, 3))
+
+
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_device_pointer_of_tiled_field_is_natural(ekx, precision):
+    """getTravelTimeFieldPointer (src/solver3d.cpp:516-525): the device
+    address holds the field in natural order, also when the solver keeps
+    it tiled, and is refreshed by every solve."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    g = ekx.Geometry3D(21, 13, 10, 10.0)
+    rng = np.random.default_rng(3)
+    s = ekx.Solver3D()
+    s.initialize(ekx.SolverOptions(
+        number_of_sweeps=2, precision=getattr(ekx.Precision, precision)), g)
+    s.set_slowness(1.0 / rng.uniform(2000.0, 4000.0, 20 * 12 * 9))
+    for src in [(35.0, 44.0, 12.0), (190.0, 5.0, 80.0)]:
+        s.set_source(src)
+        s.solve()
+        ptr, size = s.get_travel_time_field_device()
+        assert s.get_stats().tiled == 1
+        n = g.n_grid
+        host = np.empty(n, dtype=np.float64 if size == 8 else np.float32)
+        # hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)
+        assert hip.hipMemcpy(ctypes.c_void_p(host.ctypes.data),
+                             ctypes.c_void_p(ptr), ctypes.c_size_t(n * size),
+                             2) == 0
+        want = (s.get_travel_time_field() if size == 8
+                else s.get_travel_time_field_f32())
+        assert np.array_equal(host, want)
