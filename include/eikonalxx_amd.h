@@ -240,8 +240,11 @@ EKX_API ekx_status ekx_solver3d_get_travel_times(const ekx_solver3d *solver,
 /* Same field as float (the stored precision of the FP32 modes). */
 EKX_API ekx_status ekx_solver3d_get_travel_times_f32(
     const ekx_solver3d *solver, int64_t n, float *out);
-/* Device pointer of the travel-time field (borrowed until the next solve or
- * initialize), and its element size (4: float, 8: double).  A FIXED32 field
+/* Device pointer of the travel-time field in natural order (x fastest;
+ * borrowed until the next solve or initialize), and its element size (4:
+ * float, 8: double).  With the column kernel, whose fields are tiled, this
+ * is a natural-order copy made on the first request after a solve.  A
+ * FIXED32 field
  * without polish is stored as scaled integers and is not exposed here
  * (EKX_ESTATE); read it with ekx_solver3d_get_travel_times.  Corresponds to
  * Solver3D::getTravelTimeFieldPointer (src/solver3d.cpp:516-525). */
