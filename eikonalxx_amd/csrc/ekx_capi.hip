@@ -558,7 +558,7 @@ struct Engine3D
             if (modelBrickOffset[m] < 0)
             {
                 modelBrickOffset[m] = totalBricks;
-                totalBricks += nTiles*(g.nz - 1)*kBrickValues;
+                totalBricks += 4*brick_variant_values(g.nx, g.ny, g.nz);
             }
             p.sbOffset = modelBrickOffset[m];
             p.nx = g.nx;

@@ -22,6 +22,11 @@ constexpr int kColumnSkew = 2*(kColumnTile - 1);      // 14: lane a+b spread
 // are enough.
 constexpr int kColumnRing = 16;
 
+__host__ __device__ inline int tiles_of(int n)
+{
+    return (n + kColumnTile - 1)/kColumnTile;
+}
+
 // Tiled field layout of the column kernel.  The nodes of an (x, y) tile of
 // 8 x 8 form one run of nz plane records of 64 values (x fastest within a
 // record); tiles follow in row-major (tile y, tile x) order and the grid is
@@ -33,14 +38,17 @@ constexpr int kColumnRing = 16;
 // tile columns), as the current sweep leaves them for the next column in x
 // and in y.  Slowness bricks hold, per (tile, cell plane), the 9 x 9 cells
 // around the tile's nodes (cells 8 t - 1 .. 8 t + 7 per axis, clamped to the
-// grid), so a column's cells are one contiguous run too.
+// grid), so a column's cells are one contiguous run too; there are 4
+// variants, one per combination of the sweep's x and y directions, each
+// ordered as the column kernel's lanes load them.
 constexpr int kTileValues = kColumnTile*kColumnTile;    // 64
 constexpr int kEdgeValues = 2*kColumnTile + 1;           // 17
 constexpr int kBrickValues = kColumnPlane;               // 81
 
-__host__ __device__ inline int tiles_of(int n)
+// Elements of one brick variant of an nx x ny x nz node grid.
+__host__ __device__ inline int64_t brick_variant_values(int nx, int ny, int nz)
 {
-    return (n + kColumnTile - 1)/kColumnTile;
+    return (int64_t) tiles_of(nx)*tiles_of(ny)*(nz - 1)*kBrickValues;
 }
 
 // Index of node (x, y, z) in a tiled field of nTX tiles per row.

@@ -265,8 +265,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
     // kColumnRing slots plus one spare that absorbs the stores of the
     // padding steps after the last plane
-    __shared__ TS ringT[WAVES][kColumnRing + 1][kSlot];
-    __shared__ TS ringS[WAVES][kColumnRing + 1][kSlot];
+    // per slot: the travel-time plane, then the slowness plane, so that
+    // one address per lane serves both
+    __shared__ TS ring[WAVES][kColumnRing + 1][2][kSlot];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int itemIndex = blockIdx.x*WAVES + wave;
@@ -341,6 +342,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int ix = negX ? nx - 1 - jx : jx;
     const int iy = negY ? ny - 1 - jy : jy;
     const Sweep3D sw = make_sweep3d(sweep);
+    // problem constants of the Cartesian update, read once
+    const bool uniformGrid = __builtin_amdgcn_readfirstlane(p.uniform) != 0;
+    const double gridH = p.dx;
 
     // LDS plane layout: local coordinate (c, r) of the permuted frame,
     // c = jx - X0 + 1, r = jy - Y0 + 1, both in 0..8, at r*9 + c (the
@@ -358,13 +362,6 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         return lidx(1 + (negX ? kColumnTile - 1 - xl : xl),
                     1 + (negY ? kColumnTile - 1 - yl : yl));
     };
-    // Brick element e (cell 8 mx - 1 + e % 9 along x, likewise y) sits at
-    // local (e % 9, e / 9), mirrored along a reversed axis.
-    auto brickSlot = [&](int e)
-    {
-        const int cl = e % kColumnHalo, rl = e/kColumnHalo;
-        return lidx(negX ? kColumnTile - cl : cl, negY ? kColumnTile - rl : rl);
-    };
     using St = Store<TS>;
     using R = typename St::R;
     TS *Tp = T + p.tOffset;
@@ -372,8 +369,10 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // Loads per lane and step: tile element `lane`, one halo value (lanes
     // 0..8: the x-halo column c = 0, r = lane; lanes 9..16: the y-halo row
     // r = 0, c = lane - 8; the other lanes repeat lane 16's load into the
-    // ring's padding), brick elements `lane` and `lane + 64` (lanes past
-    // the 81 cells repeat their first element into the padding).
+    // ring's padding), and brick elements `lane` and 64 + min(lane, 16):
+    // the brick variant of the sweep's x and y directions stores its cells
+    // in that lane order, so they go to the same ring positions as the
+    // travel times.
     // Every prefetch load is unconditional from an in-bounds address;
     // conditional loads would make the compiler wait for them at the merge
     // point and serialise the prefetch.
@@ -382,9 +381,6 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int dstHalo = lane < kColumnHalo ? lidx(0, lane)
                       : (lane < kEdgeValues ? lidx(lane - kColumnTile, 0)
                                             : kColumnPlane);
-    const bool has1 = lane + 64 < kColumnPlane;
-    const int dstS0 = brickSlot(lane);
-    const int dstS1 = has1 ? brickSlot(lane + 64) : kColumnPlane;
     // Halo source: the upwind column's edge records (when that column ran
     // in this sweep: its last node X0 - 1 is at or past the octant start),
     // else the node itself in the tile records.  Per lane: the address at
@@ -435,16 +431,17 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         const int az = negZ ? nz - 1 - jz : jz;
         return (char *) (T + p.eOffset + (tileIndex*nz + az)*kEdgeValues);
     };
+    const TS *brick = S + p.sbOffset + (negX + 2*negY)*
+                          brick_variant_values(nx, ny, nz);
     auto planeS = [&](int q) -> const char *
     {
         const int pcz = min(P0 + q, ncz - 1);
         const int cz = negZ ? ncz - 1 - pcz : pcz;
-        return (const char *) (S + p.sbOffset +
-                               (tileIndex*ncz + cz)*kBrickValues);
+        return (const char *) (brick + (tileIndex*ncz + cz)*kBrickValues);
     };
     const uint32_t tB = (uint32_t) lane*sizeof(TS);
     const uint32_t sB0 = (uint32_t) lane*sizeof(TS);
-    const uint32_t sB1 = (uint32_t) (has1 ? lane + 64 : lane)*sizeof(TS);
+    const uint32_t sB1 = (uint32_t) (kTileValues + h)*sizeof(TS);
     auto at = [](const char *base, uint32_t off) -> TS
     {
         return *(const TS *) (base + off);
@@ -505,8 +502,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // steps only repeat idempotent work: in-bounds loads, no computation,
     // and a re-store of the last plane's final values).
     const int nSteps = ((nPlanes + kColumnSkew + 1 + DL - 1)/DL)*DL;
-    TS(*myT)[kSlot] = ringT[wave];
-    TS(*myS)[kSlot] = ringS[wave];
+    TS(*myR)[2][kSlot] = ring[wave];
     for (int t0 = 0; t0 < nSteps; t0 += DL)
     {
 #pragma unroll
@@ -519,12 +515,12 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             {
                 const int slot = tc < nPlanes ? (tc & (kColumnRing - 1))
                                               : kColumnRing;
-                TS *dT = myT[slot];
-                TS *dS = myS[slot];
+                TS *dT = myR[slot][0];
+                TS *dS = myR[slot][1];
                 dT[dstOwn] = bT0[u];
-                dS[dstS0] = bS0[u];
+                dS[dstOwn] = bS0[u];
                 dT[dstHalo] = bT1[u];
-                dS[dstS1] = bS1[u];
+                dS[dstHalo] = bS1[u];
             }
             // (b) prefetch plane tc + DL into the freed registers (past the
             // last plane the load repeats it and is never stored)
@@ -538,8 +534,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             }
             // (d, read) the plane the last lane finished a step ago; read
             // with the neighbour values below, stored after the update
-            const TS wbValue = myT[wbQ & (kColumnRing - 1)][dstOwn];
-            const TS wbEdgeValue = myT[wbQ & (kColumnRing - 1)][wbEdge];
+            const TS wbValue = myR[wbQ & (kColumnRing - 1)][0][dstOwn];
+            const TS wbEdgeValue = myR[wbQ & (kColumnRing - 1)][0][wbEdge];
             // (c) every lane advances its node on plane tc - (a + b).  The
             // reads and the register shift run on every lane at every step
             // (outside the lane's planes they read unused ring slots and
@@ -548,8 +544,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // merging them at a branch.
             const int q = tc - (a + b);
             {
-                TS *sT = myT[q & (kColumnRing - 1)] + oA;
-                const TS *sS = myS[q & (kColumnRing - 1)] + oA;
+                TS *sT = myR[q & (kColumnRing - 1)][0] + oA;
+                const TS *sS = myR[q & (kColumnRing - 1)][1] + oA;
                 const TS ownCode = sT[kD];
                 R tOwn = St::time(ownCode, p);
                 const R t1 = St::time(sT[kC], p);
@@ -559,29 +555,11 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 const R c10 = St::slowness(sS[kB], p);
                 const R c01 = St::slowness(sS[kC], p);
                 const R c11 = St::slowness(sS[kD], p);
-                if (upd && q > 0 && q < nPlanes)
+                // the wave-uniform choice of operator outside the lanes'
+                // branch, so that it stays a scalar branch
+                const bool live = upd && q > 0 && q < nPlanes;
+                auto keep = [&](R tUpd)
                 {
-                    const int jz = P0 + q;
-                    const int iz = negZ ? nz - 1 - jz : jz;
-                    R tUpd;
-                    if (FACTORED || (kMerged && boxColumn))
-                    {
-                        tUpd = local_update3d<R>(
-                            p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00,
-                            c10, c01, t1, t2, t3, pt4, pt5, pt6, pt7);
-                    }
-                    else if (p.uniform)
-                    {
-                        tUpd = cartesian_uniform3d<R>(
-                            (R) p.dx, ps00, ps10, ps11, ps01, c00, c10, c01,
-                            t1, t2, t3, pt4, pt5, pt6, pt7);
-                    }
-                    else
-                    {
-                        tUpd = cartesian_nonuniform3d<R>(
-                            p, ps00, ps10, ps11, ps01, c00, c10, c01, t1, t2,
-                            t3, pt4, pt5, pt6, pt7);
-                    }
                     const TS c = St::code(tUpd, p);
                     if (c < ownCode)
                     {
@@ -590,6 +568,32 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                         tOwn = tNew;
                         sT[kD] = c;
                     }
+                };
+                if (FACTORED || (kMerged && boxColumn))
+                {
+                    if (live)
+                    {
+                        const int jz = P0 + q;
+                        const int iz = negZ ? nz - 1 - jz : jz;
+                        keep(local_update3d<R>(
+                            p, sw, ix, iy, iz, ps00, ps10, ps11, ps01, c00,
+                            c10, c01, t1, t2, t3, pt4, pt5, pt6, pt7));
+                    }
+                }
+                else if (uniformGrid)
+                {
+                    if (live)
+                    {
+                        keep(cartesian_uniform3d<R>(
+                            (R) gridH, ps00, ps10, ps11, ps01, c00, c10, c01,
+                            t1, t2, t3, pt4, pt5, pt6, pt7));
+                    }
+                }
+                else if (live)
+                {
+                    keep(cartesian_nonuniform3d<R>(
+                        p, ps00, ps10, ps11, ps01, c00, c10, c01, t1, t2, t3,
+                        pt4, pt5, pt6, pt7));
                 }
                 pt4 = tOwn;
                 pt5 = t1;
@@ -642,9 +646,14 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
 // ---------------------------------------------------------------------------
 // Tiled layout: slowness bricks and natural-order copies of fields.
 // ---------------------------------------------------------------------------
-// Brick element e of (tile (tx, ty), cell plane cz) = cell
-// (clamp(8 tx - 1 + e % 9), clamp(8 ty - 1 + e / 9), cz) of the natural
-// cell array.
+// The 4 brick variants of one model (v = negX + 2 negY: the x and y
+// directions of a sweep), each ncz records per tile in the column kernel's
+// lane order: element l < 64 is the cell at the ring position of tile
+// element l (local (1 + x, 1 + y), mirrored along a reversed axis),
+// element 64 + h the cell at the halo position of halo lane h (h < 9:
+// local (0, h); else (h - 8, 0)).  The cell at local (c, r) is grid cell
+// (8 tx - 1 + c', 8 ty - 1 + r') with c' = c, or 8 - c along a reversed
+// axis, clamped into the grid.
 template <typename TS>
 __global__ void __launch_bounds__(256)
 tile_slowness_kernel(int nx, int ny, int nz, const TS *__restrict__ s,
@@ -652,19 +661,36 @@ tile_slowness_kernel(int nx, int ny, int nz, const TS *__restrict__ s,
 {
     const int ncx = nx - 1, ncy = ny - 1, ncz = nz - 1;
     const int nTX = tiles_of(nx);
-    const int64_t n = (int64_t) nTX*tiles_of(ny)*ncz*kBrickValues;
+    const int64_t perVariant = brick_variant_values(nx, ny, nz);
+    const int64_t n = 4*perVariant;
     for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
          i += (int64_t) gridDim.x*blockDim.x)
     {
-        const int64_t rec = i/kBrickValues;
-        const int e = (int) (i - rec*kBrickValues);
+        const int v = (int) (i/perVariant);
+        const int64_t j = i - v*perVariant;
+        const int64_t rec = j/kBrickValues;
+        const int l = (int) (j - rec*kBrickValues);
         const int tile = (int) (rec/ncz);
         const int cz = (int) (rec - (int64_t) tile*ncz);
         const int ty = tile/nTX, tx = tile - ty*nTX;
-        const int cx = min(max(tx*kColumnTile - 1 + e % kColumnHalo, 0),
-                           ncx - 1);
-        const int cy = min(max(ty*kColumnTile - 1 + e/kColumnHalo, 0),
-                           ncy - 1);
+        const bool negX = v & 1, negY = v & 2;
+        int c, r;
+        if (l < kTileValues)
+        {
+            const int xl = l % kColumnTile, yl = l/kColumnTile;
+            c = 1 + (negX ? kColumnTile - 1 - xl : xl);
+            r = 1 + (negY ? kColumnTile - 1 - yl : yl);
+        }
+        else
+        {
+            const int hl = l - kTileValues;
+            c = hl < kColumnHalo ? 0 : hl - kColumnTile;
+            r = hl < kColumnHalo ? hl : 0;
+        }
+        const int cl = negX ? kColumnTile - c : c;
+        const int rl = negY ? kColumnTile - r : r;
+        const int cx = min(max(tx*kColumnTile - 1 + cl, 0), ncx - 1);
+        const int cy = min(max(ty*kColumnTile - 1 + rl, 0), ncy - 1);
         b[i] = s[((int64_t) cz*ncy + cy)*ncx + cx];
     }
 }
@@ -917,8 +943,7 @@ template <typename TS>
 hipError_t launch_tile_slowness(int nx, int ny, int nz, const TS *s, TS *b,
                                 hipStream_t stream)
 {
-    const int64_t n = (int64_t) tiles_of(nx)*tiles_of(ny)*(nz - 1)*
-                      kBrickValues;
+    const int64_t n = 4*brick_variant_values(nx, ny, nz);
     hipLaunchKernelGGL(tile_slowness_kernel<TS>,
                        dim3(grid_for(n, 256, 16384)), dim3(256), 0, stream,
                        nx, ny, nz, s, b);
