@@ -672,7 +672,8 @@ struct Engine3D
         // Column-chunk work lists by chain, then diagonal bx + by + bz.
         int maxNz = 0;
         for (int i = 0; i < n; ++i) { maxNz = std::max(maxNz, geoms[i].nz); }
-        chunkH = o.chunk_planes > 0 ? std::min(o.chunk_planes, maxNz) : maxNz;
+        chunkH = o.chunk_planes > 0 ? std::min(o.chunk_planes, maxNz)
+                                    : auto_chunk(geoms, maxNz);
         nDiag = 0;
         for (int i = 0; i < n; ++i)
         {
@@ -734,6 +735,52 @@ struct Engine3D
                                hipMemcpyHostToDevice, stream));
         EKX_HIP(hipStreamSynchronize(stream));
         return EKX_OK;
+    }
+
+    // Planes per z-chunk when the caller leaves the choice
+    // (chunk_planes = 0), by size class: the longest of whole columns, 96,
+    // 64, 48, 32, 24, 16, 12 or 8 planes whose column diagonals hold on
+    // average at least 1.75 times the column waves the device keeps
+    // resident.
+    // Shorter chunks put more column-chunks on each diagonal (a sweep's
+    // dependency chain is nbx + nby + nbz - 2 launches) at the price of
+    // the 15-step lane skew at both ends of every chunk.  Measured on the
+    // MI355X: the 256-source 385 x 385 x 193 tables run fastest with whole
+    // columns, a 64-problem batch of 96..320-point grids with 32 planes, a
+    // single 257 x 257 x 193 or 129^3 grid with 8 (4.5x and 3x faster than
+    // whole columns); every choice gives the same field.
+    int auto_chunk(const std::vector<ekx_geometry3d> &geoms, int maxNz) const
+    {
+        int device = 0, cus = 256;
+        if (hipGetDevice(&device) == hipSuccess)
+        {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount,
+                                      device) == hipSuccess && v > 0)
+            {
+                cus = v;
+            }
+        }
+        // resident column waves per CU: LDS rings of 22.8 KB (FP64) or
+        // 11.4 KB (4-byte elements) per wave
+        const int perCU = sweeps_f64() ? 7 : 14;
+        const double want = 1.75*cus*perCU;
+        static const int kCandidates[] = {0, 96, 64, 48, 32, 24, 16, 12, 8};
+        for (int c : kCandidates)
+        {
+            const int h = c == 0 ? maxNz : std::min(c, maxNz);
+            double items = 0;
+            int diags = 1;
+            for (const ekx_geometry3d &g : geoms)
+            {
+                const int nbx = tiles_of(g.nx), nby = tiles_of(g.ny);
+                const int nbz = (g.nz + h - 1)/h;
+                items += (double) nbx*nby*nbz;
+                diags = std::max(diags, nbx + nby + nbz - 2);
+            }
+            if (items/diags >= want) { return h; }
+        }
+        return std::min(8, maxNz);
     }
 
     // Uploads one model (cell slowness, double) to its slot.

@@ -122,8 +122,11 @@ typedef struct
     int chunk_planes;              /* column kernel: z-planes per chunk of
                                       a column (chunks run on diagonals
                                       bx+by+bz, shortening the dependency
-                                      chain of a sweep); 0 (default):
-                                      whole columns                        */
+                                      chain of a sweep); 0 (default): chosen
+                                      by size class (whole columns for
+                                      large batches, down to 8 planes for
+                                      one grid); results do not depend on
+                                      it                                   */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */
