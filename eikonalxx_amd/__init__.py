@@ -106,6 +106,8 @@ class SolveStats(ctypes.Structure):
                 ("timed_updates", ctypes.c_int64),
                 ("iterations_run", ctypes.c_int32),
                 ("tiled", ctypes.c_int32),
+                ("chunk_planes", ctypes.c_int32),
+                ("chains", ctypes.c_int32),
                 ("timed_kernel_ms", ctypes.c_double),
                 ("last_iteration_change", ctypes.c_double),
                 ("last_solve_ms", ctypes.c_double),
@@ -166,6 +168,7 @@ EXPORTED_SYMBOLS = [
     "ekx_solver2d_have_travel_time_gradient_field",
     "ekx_solver2d_get_travel_time_gradient_field",
     "ekx_model3d_layered_gaussian", "ekx_solver3d_set_layered_gaussian_model",
+    "ekx_solver3d_autotune", "ekx_batch3d_autotune",
 ]
 
 _lib = None
@@ -260,6 +263,9 @@ def _declare(L):
     L.ekx_solver2d_compute_travel_time_gradient_field.argtypes = [_P]
     L.ekx_solver2d_have_travel_time_gradient_field.argtypes = [_P]
     L.ekx_solver2d_get_travel_time_gradient_field.argtypes = [_P, _I64, _PD]
+    for name in ("ekx_solver3d_autotune", "ekx_batch3d_autotune"):
+        getattr(L, name).argtypes = [_P, ctypes.POINTER(_I),
+                                     ctypes.POINTER(_I)]
     L.ekx_model3d_layered_gaussian.argtypes = [
         ctypes.POINTER(_Geometry3D), ctypes.POINTER(_LayeredGaussian), _I64,
         _PD]
@@ -503,6 +509,15 @@ class Solver3D:
     def solve(self):
         _check(self._L.ekx_solver3d_solve(self._h))
 
+    def autotune(self):
+        """Times short trial solves over candidate launch shapes and keeps
+        the fastest (no reference counterpart); returns (chunk_planes,
+        chains).  Fields do not depend on the shape."""
+        c, n = _I(), _I()
+        _check(self._L.ekx_solver3d_autotune(self._h, ctypes.byref(c),
+                                             ctypes.byref(n)))
+        return c.value, n.value
+
     def have_travel_time_field(self):
         return bool(self._L.ekx_solver3d_have_travel_time_field(self._h))
 
@@ -633,6 +648,14 @@ class Batch3D:
 
     def solve(self):
         _check(self._L.ekx_batch3d_solve(self._h))
+
+    def autotune(self):
+        """The launch-shape autotuner for the whole batch; returns
+        (chunk_planes, chains)."""
+        c, n = _I(), _I()
+        _check(self._L.ekx_batch3d_autotune(self._h, ctypes.byref(c),
+                                            ctypes.byref(n)))
+        return c.value, n.value
 
     def get_travel_time_field(self, problem):
         n = self._geoms[problem].n_grid

@@ -23,6 +23,9 @@ namespace
 
 // Blocks of the model-maximum reduction (FIXED32 modes).
 constexpr size_t kMaxPartials = 1024;
+// Column z-chunk depths the launch-shape choice considers (0: whole
+// columns), longest first.
+constexpr int kChunkCandidates[] = {0, 96, 64, 48, 32, 24, 16, 12, 8};
 // Independent launch chains of a batch (one stream each).
 constexpr int kMaxChains = 16;
 
@@ -637,9 +640,26 @@ struct Engine3D
             }
         }
         else { change.release(); }
+        EKX_CHECK(build_schedule());
+        EKX_HIP(hipStreamSynchronize(stream));
+        return EKX_OK;
+    }
+
+    // The launch schedule of the current options: chains, the z-chunk
+    // depth and the column-chunk work lists by chain and diagonal.
+    ekx_status build_schedule()
+    {
         // Chains: contiguous ranges of problems with about equal work
         // (node count).  One chain unless the batch is large enough to
         // keep several independent launch sequences busy.
+        const ekx_solver_options &o = opt;
+        const std::vector<ekx_geometry3d> &geoms = geometry;
+        const int n = (int) geoms.size();
+        int64_t naturalNodes = 0;
+        for (const ekx_geometry3d &g : geoms)
+        {
+            naturalNodes += (int64_t) g.nx*g.ny*g.nz;
+        }
         nChains = o.chains > 0 ? std::min(o.chains, n)
                                : (n >= 16 ? std::min(4, n) : 1);
         if (kernel() != EKX_KERNEL_COLUMN) { nChains = 1; }
@@ -652,7 +672,7 @@ struct Engine3D
                 done += (int64_t) g.nx*g.ny*g.nz;
                 int c = (int) chainBegin.size();      // chain being filled
                 if (c < nChains &&
-                    done*nChains >= totalNodes*(int64_t) c && i + 1 < n)
+                    done*nChains >= naturalNodes*(int64_t) c && i + 1 < n)
                 {
                     chainBegin.push_back(i + 1);
                 }
@@ -733,7 +753,6 @@ struct Engine3D
         EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
                                flat.size()*sizeof(ColumnItem),
                                hipMemcpyHostToDevice, stream));
-        EKX_HIP(hipStreamSynchronize(stream));
         return EKX_OK;
     }
 
@@ -765,8 +784,7 @@ struct Engine3D
         // 11.4 KB (4-byte elements) per wave
         const int perCU = sweeps_f64() ? 7 : 14;
         const double want = 1.75*cus*perCU;
-        static const int kCandidates[] = {0, 96, 64, 48, 32, 24, 16, 12, 8};
-        for (int c : kCandidates)
+        for (int c : kChunkCandidates)
         {
             const int h = c == 0 ? maxNz : std::min(c, maxNz);
             double items = 0;
@@ -1258,6 +1276,133 @@ struct Engine3D
         return EKX_OK;
     }
 
+    // One short solve for the autotuner: the initialisation sweeps and one
+    // refinement iteration, run as later solves run (the captured graph,
+    // unless use_graph is off): once to capture, then timed; device ms.
+    template <typename TS>
+    ekx_status trial_run(TS *T, const TS *S, float &ms)
+    {
+        const int nProb = (int) problems.size();
+        ekx_solve_stats st{};
+        timing = false;
+        for (int pass = 0; pass < 2; ++pass)
+        {
+            EKX_HIP(hipEventRecord(evStart, stream));
+            EKX_HIP(launch_init3d<TS>(dProblems.ptr, nProb, maxCount, T,
+                                      stream));
+            EKX_CHECK(run_segment<TS>(0, T, S, true, 1, st));
+            EKX_HIP(hipEventRecord(evStop, stream));
+            EKX_HIP(hipEventSynchronize(evStop));
+        }
+        EKX_HIP(hipEventElapsedTime(&ms, evStart, evStop));
+        return EKX_OK;
+    }
+
+    ekx_status trial_ms(float &ms)
+    {
+        EKX_CHECK(prepare_sources());
+        if (sweeps_f64()) { return trial_run<double>(T64.ptr, S64.ptr, ms); }
+        if (fixed32()) { return trial_run<uint32_t>(TQ.ptr, SQ.ptr, ms); }
+        return trial_run<float>(T32.ptr, S32.ptr, ms);
+    }
+
+    // Measured choice of the launch shape (no reference counterpart): times
+    // a short solve for z-chunk depths around the size-class default (the
+    // default and its neighbours in auto_chunk's list) and 1, 2 or 4
+    // chains, and keeps the fastest for later solves.  Fields do not
+    // depend on the choice; the field of the last solve is discarded.
+    ekx_status autotune(int &chunkOut, int &chainsOut)
+    {
+        if (!haveModel)
+        {
+            return fail(EKX_ESTATE, "Velocity model not set");
+        }
+        for (size_t i = 0; i < problems.size(); ++i)
+        {
+            if (!haveSource[i])
+            {
+                return fail(EKX_ESTATE, "Source location not set");
+            }
+        }
+        if (kernel() == EKX_KERNEL_COLUMN)
+        {
+            int maxNz = 0;
+            for (const ekx_geometry3d &g : geometry)
+            {
+                maxNz = std::max(maxNz, g.nz);
+            }
+            std::vector<int> depths;
+            for (int c : kChunkCandidates)
+            {
+                const int h = c == 0 ? maxNz : std::min(c, maxNz);
+                if (depths.empty() || h < depths.back()) { depths.push_back(h); }
+            }
+            const int h0 = auto_chunk(geometry, maxNz);
+            size_t k0 = 0;
+            while (k0 + 1 < depths.size() && depths[k0] != h0) { ++k0; }
+            std::vector<int> tryDepth;
+            for (size_t k = (k0 > 0 ? k0 - 1 : 0);
+                 k <= std::min(k0 + 1, depths.size() - 1); ++k)
+            {
+                tryDepth.push_back(depths[k]);
+            }
+            const int n = (int) problems.size();
+            std::vector<int> tryChains;
+            for (int c : {1, 2, 4})
+            {
+                if (c <= n) { tryChains.push_back(c); }
+            }
+            const ekx_solver_options saved = opt;
+            float best = 0;
+            int bestDepth = h0, bestChains = 1;
+            bool first = true;
+            // the default shape first; another replaces the best only when
+            // it is more than 3% faster (trial times vary by about that)
+            const int c0 = saved.chains > 0 ? std::min(saved.chains, n)
+                                            : (n >= 16 ? std::min(4, n) : 1);
+            std::vector<std::pair<int, int>> shapes{{h0, c0}};
+            for (int d : tryDepth)
+            {
+                for (int c : tryChains)
+                {
+                    if (d != h0 || c != c0) { shapes.push_back({d, c}); }
+                }
+            }
+            for (const auto &shape : shapes)
+            {
+                const int d = shape.first, c = shape.second;
+                opt.chunk_planes = d;
+                opt.chains = c;
+                drop_graph();
+                ekx_status rc = build_schedule();
+                float ms = 0;
+                if (rc == EKX_OK) { rc = trial_ms(ms); }
+                if (rc != EKX_OK)
+                {
+                    opt = saved;
+                    drop_graph();
+                    (void) build_schedule();
+                    return rc;
+                }
+                if (first || ms < 0.97f*best)
+                {
+                    best = ms;
+                    bestDepth = d;
+                    bestChains = c;
+                    first = false;
+                }
+            }
+            opt.chunk_planes = bestDepth;
+            opt.chains = bestChains;
+            drop_graph();
+            EKX_CHECK(build_schedule());
+        }
+        haveField = false;
+        chunkOut = chunkH;
+        chainsOut = nChains;
+        return EKX_OK;
+    }
+
     ekx_status solve()
     {
         timing = false;
@@ -1350,6 +1495,8 @@ struct Engine3D
             // timed_updates: counted per timed sweep by count_sweep
         }
         st.tiled = tiled ? 1 : 0;
+        st.chunk_planes = kernel() == EKX_KERNEL_COLUMN ? chunkH : 0;
+        st.chains = nChains;
         stats = st;
         haveField = true;
         solveStamp += 1;
@@ -1542,6 +1689,10 @@ struct Engine3D
 
     ekx_status copy_field(int problem, int64_t n, double *out) const
     {
+        if (!haveField)
+        {
+            return fail(EKX_ESTATE, "Travel time field not yet computed");
+        }
         const ekx_geometry3d &g = geometry[problem];
         int64_t nGrid = (int64_t) g.nx*g.ny*g.nz;
         if (n != nGrid)
@@ -1984,6 +2135,17 @@ ekx_status ekx_solver3d_solve(ekx_solver3d *h)
     return h->engine.solve();
 }
 
+ekx_status ekx_solver3d_autotune(ekx_solver3d *h, int *chunk_planes,
+                                 int *chains)
+{
+    EKX_CHECK(require_initialized(h));
+    int c = 0, n = 0;
+    EKX_CHECK(h->engine.autotune(c, n));
+    if (chunk_planes) { *chunk_planes = c; }
+    if (chains) { *chains = n; }
+    return EKX_OK;
+}
+
 int ekx_solver3d_have_travel_time_field(const ekx_solver3d *h)
 {
     return h && h->initialized && h->engine.haveField ? 1 : 0;
@@ -2246,6 +2408,16 @@ ekx_status ekx_batch3d_solve(ekx_batch3d *h)
 {
     EKX_CHECK(require_batch(h));
     return h->engine.solve();
+}
+
+ekx_status ekx_batch3d_autotune(ekx_batch3d *h, int *chunk_planes, int *chains)
+{
+    EKX_CHECK(require_batch(h));
+    int c = 0, n = 0;
+    EKX_CHECK(h->engine.autotune(c, n));
+    if (chunk_planes) { *chunk_planes = c; }
+    if (chains) { *chains = n; }
+    return EKX_OK;
 }
 
 ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *h, int problem,

@@ -162,6 +162,10 @@ typedef struct
     int32_t tiled;                /* 1: fields in the column kernel's tiled
                                      layout (device copies in natural
                                      order are made on request)           */
+    int32_t chunk_planes;         /* column kernel: z-planes per column
+                                     chunk used (the size-class choice when
+                                     the option is 0)                     */
+    int32_t chains;               /* launch chains used                    */
     double timed_kernel_ms;       /* summed durations of the timed launches */
     double last_iteration_change; /* early_exit: largest decrease of one
                                      update in the last refinement
@@ -234,6 +238,15 @@ EKX_API ekx_status ekx_solver3d_get_source_cell(const ekx_solver3d *solver,
                                                 int cell[3], int *cell_index);
 /* Solver3D::solve (solver3d.hpp:127, src/solver3d.cpp:391-500). */
 EKX_API ekx_status ekx_solver3d_solve(ekx_solver3d *solver);
+/* No reference counterpart; for callers that solve many times on one
+ * setup.  Times short trial solves (the initialisation sweeps and one
+ * refinement iteration) for column z-chunk depths around the size-class
+ * default and for 1, 2 or 4 launch chains, and keeps the fastest launch
+ * shape for later solves.  Fields do not depend on the shape.  Needs the
+ * model and the source set; discards the current field.  chunk_planes and
+ * chains (each may be NULL) receive the choice. */
+EKX_API ekx_status ekx_solver3d_autotune(ekx_solver3d *solver,
+                                         int *chunk_planes, int *chains);
 /* Solver3D::haveTravelTimeField (src/solver3d.cpp:510-514). */
 EKX_API int ekx_solver3d_have_travel_time_field(const ekx_solver3d *solver);
 /* Solver3D::getTravelTimeField (solver3d.hpp:137, src/solver3d.cpp:
@@ -351,6 +364,12 @@ EKX_API ekx_status ekx_batch3d_initialize(ekx_batch3d *batch,
 EKX_API ekx_status ekx_batch3d_set_source(ekx_batch3d *batch, int problem,
                                           double x, double y, double z);
 EKX_API ekx_status ekx_batch3d_solve(ekx_batch3d *batch);
+/* The launch-shape autotuner of ekx_solver3d_autotune for a batch (no
+ * reference counterpart): candidate z-chunk depths around the size-class
+ * default times 1, 2 or 4 chains, each timed on a short solve of the whole
+ * batch; the fastest is kept. */
+EKX_API ekx_status ekx_batch3d_autotune(ekx_batch3d *batch,
+                                        int *chunk_planes, int *chains);
 EKX_API ekx_status ekx_batch3d_get_travel_times(const ekx_batch3d *batch,
                                                 int problem, int64_t n,
                                                 double *out);
