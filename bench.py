@@ -92,6 +92,93 @@ def cpu_baseline(w, slow, source, threads, iterations):
     return upd / dt, dt, upd
 
 
+def secondary_workloads(ekx):
+    """The other 3D workloads of the benchmark description, timed in FP64
+    and FP32_POLISH (updates/s of the best of 2 solves after a warmup; max
+    and RMS relative error of FP32_POLISH against FP64 on the same run):
+    one 257 x 257 x 193 grid (12 layers 2000-7000 m/s x (1 + 0.05 G),
+    l = 250 m, radius 5, 3 iterations), and 64 independent problems with
+    96..320 points per axis (h = 20 m, the same model family, one model
+    each).  Models come from the library's GPU generator (seeded)."""
+    from eikonalxx_amd import workloads
+    out = {}
+
+    def timed(solve, stats):
+        solve()
+        best = None
+        for _ in range(2):
+            solve()
+            ms = stats().last_solve_ms
+            best = ms if best is None else min(best, ms)
+        return stats().updates / (best / 1e3), best
+
+    # one grid
+    nx, ny, nz, h = 257, 257, 193, 25.0
+    rng = np.random.default_rng(3003)
+    layers = np.sort(rng.uniform(2000.0, 7000.0, 12))
+    model = ekx.LayeredGaussianModel(layers, 0.05, 250.0, 3003)
+    src = tuple(float(rng.uniform(5 * h, (n - 6) * h)) for n in (nx, ny, nz))
+    g = ekx.Geometry3D(nx, ny, nz, h)
+    fields, entry = {}, {}
+    for mode in ("FP64", "FP32_POLISH"):
+        s = ekx.Solver3D()
+        s.initialize(ekx.SolverOptions(
+            number_of_sweeps=3, factored_radius=5,
+            precision=getattr(ekx.Precision, mode)), g)
+        s.set_layered_gaussian_model(model)
+        s.set_source(src)
+        rate, ms = timed(s.solve, s.get_stats)
+        fields[mode] = s.get_travel_time_field()
+        entry[mode] = {"updates_per_s": rate, "ms_per_solve": ms,
+                       "chunk_planes": s.get_stats().chunk_planes}
+        del s
+    # error floor s h with the smallest slowness of the model
+    s_min = 1.0 / (layers[-1] * 1.25)
+    mx, rms = rel_error(fields["FP32_POLISH"], fields["FP64"], s_min, h)
+    entry["FP32_POLISH"].update(max_rel_err_vs_fp64=mx, rms_rel_err_vs_fp64=rms)
+    out["single_257x257x193"] = entry
+    log("single 257x257x193: FP64 %.3e, FP32_POLISH %.3e updates/s "
+        "(max err %.2e)" % (entry["FP64"]["updates_per_s"],
+                           entry["FP32_POLISH"]["updates_per_s"], mx))
+
+    # 64 uneven problems
+    problems = []
+    for i in range(64):
+        r = np.random.default_rng(4004 + i)
+        dims = tuple(int(v) for v in r.integers(96, 321, size=3))
+        gi = ekx.Geometry3D(*dims, 20.0)
+        lv = np.sort(r.uniform(2000.0, 7000.0, 12))
+        v = ekx.LayeredGaussianModel(lv, 0.05, 250.0, 5000 + i).generate(gi)
+        slow = workloads.cell_slowness(v, *dims)
+        srci = tuple(float(r.uniform(2 * 20.0, (n - 3) * 20.0))
+                     for n in dims)
+        problems.append((gi, slow, srci))
+    fields, entry = {}, {}
+    picks = (0, 21, 42, 63)
+    for mode in ("FP64", "FP32_POLISH"):
+        b = ekx.Batch3D()
+        b.initialize(ekx.SolverOptions(
+            number_of_sweeps=3, factored_radius=5,
+            precision=getattr(ekx.Precision, mode)), problems)
+        rate, ms = timed(b.solve, b.get_stats)
+        fields[mode] = [b.get_travel_time_field(i) for i in picks]
+        st = b.get_stats()
+        entry[mode] = {"updates_per_s": rate, "ms_per_solve": ms,
+                       "chunk_planes": st.chunk_planes, "chains": st.chains}
+        del b
+    errs = [rel_error(a, f, problems[i][1].min(), 20.0)
+            for a, f, i in zip(fields["FP32_POLISH"], fields["FP64"], picks)]
+    entry["FP32_POLISH"].update(
+        max_rel_err_vs_fp64=max(e[0] for e in errs),
+        rms_rel_err_vs_fp64=float(np.sqrt(np.mean([e[1] ** 2
+                                                   for e in errs]))))
+    out["batch64_uneven_96_320"] = entry
+    log("64 uneven problems: FP64 %.3e, FP32_POLISH %.3e updates/s"
+        % (entry["FP64"]["updates_per_s"],
+           entry["FP32_POLISH"]["updates_per_s"]))
+    return out
+
+
 def measured_traffic(kernel, per_launch, args):
     """HBM bytes per launch of `kernel`, from the rocprofv3 counter summary
     under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
@@ -293,6 +380,8 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the eager per-launch timing solve (the "
                          "roofline then uses the whole sweep phase)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the single-grid and uneven-batch workloads")
     ap.add_argument("--skip-checks", action="store_true",
                     help="time the batched solves only: no accuracy check "
                          "against FP64 and no plain single-source solve "
@@ -374,6 +463,11 @@ def main():
             % (min(plain_ms), plain_rate))
         del plain
 
+    # ---- the other 3D workloads ---------------------------------------------
+    secondary = None
+    if not args.skip_checks and not args.no_secondary:
+        secondary = secondary_workloads(ekx)
+
     # ---- CPU baseline (oracle, bounded sample) ------------------------------
     cpu = None
     if not args.no_cpu and not args.skip_checks:
@@ -436,6 +530,7 @@ def main():
             "speedup_vs_fp64_plain": (hr["updates_per_s"] / plain_rate
                                       if plain_rate else None),
             "modes": summary,
+            "other_workloads": secondary,
         },
         "roofline": hr["roofline"],
         "cpu_baseline": cpu,
