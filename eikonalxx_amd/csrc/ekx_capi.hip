@@ -379,12 +379,10 @@ struct Engine3D
     std::vector<int> chainBegin;
     std::vector<hipStream_t> chainStreams;
     std::vector<hipEvent_t> chainEvents;
-    // Upper bound, over any source position, of the columns per diagonal
-    // that overlap one problem's factored box: the box spans at most
-    // ceil((2r+1)/8)+1 columns per axis.
-    // Upper bound of the chunks per problem that overlap the factored box
-    // on one diagonal: the box's columns in x times those in y (each
-    // (bx, by) meets a diagonal at one bz at most).
+    // Upper bound, over any source position, of the column chunks of one
+    // problem that overlap its factored box on one diagonal: the box spans
+    // at most ceil((2r+1)/8)+1 columns per axis, and each (bx, by) meets a
+    // diagonal at one bz at most.
     int boxPerDiag = 0;
     hipStream_t stream = nullptr;
     hipStream_t sideStream = nullptr;   // factored-box columns
