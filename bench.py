@@ -179,6 +179,32 @@ def secondary_workloads(ekx):
     return out
 
 
+def field_kernels(ekx, args):
+    """A run for profiling the field kernels: 16 sources of the tables
+    workload (1 iteration), 4096 surface stations, and per solve the
+    gradient and the natural-order copy of 3 problems."""
+    from eikonalxx_amd import workloads
+    w = workloads.config5_tables(nx=args.nx, ny=args.ny, nz=args.nz,
+                                 h=args.h, n_sources=16)
+    slow = workloads.cell_slowness(w["velocity"], w["nx"], w["ny"], w["nz"])
+    g = ekx.Geometry3D(w["nx"], w["ny"], w["nz"], w["h"])
+    b = ekx.Batch3D()
+    b.initialize(ekx.SolverOptions(number_of_sweeps=1, factored_radius=3),
+                 [(g, slow, s) for s in w["sources"]])
+    rng = np.random.default_rng(1)
+    b.set_stations(rng.random((4096, 3)) *
+                   np.array([(w["nx"] - 1) * w["h"], (w["ny"] - 1) * w["h"],
+                             0.0]))
+    for _ in range(3):
+        b.solve()
+        for k in range(3):
+            b.get_travel_time_gradient_field(k)
+        for k in range(3):
+            b.get_travel_time_field(k)
+    log("field kernels: 3 solves of 16 sources, 9 gradients, 9 copies")
+    return 0
+
+
 def measured_traffic(kernel, per_launch, args):
     """HBM bytes per launch of `kernel`, from the rocprofv3 counter summary
     under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
@@ -380,6 +406,10 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the eager per-launch timing solve (the "
                          "roofline then uses the whole sweep phase)")
+    ap.add_argument("--field-kernels", action="store_true",
+                    help="only run the field-kernel workload (gradients, "
+                         "station table, natural-order copies) for "
+                         "profiling; prints no JSON line")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the single-grid and uneven-batch workloads")
     ap.add_argument("--skip-checks", action="store_true",
@@ -404,6 +434,8 @@ def main():
     if ekx.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    if args.field_kernels:
+        return field_kernels(ekx, args)
     if args.copy_calibration > 0 and copy_calibration(
             args.copy_calibration) != 0:
         return 1
