@@ -757,7 +757,7 @@ struct Engine3D
     // Planes per z-chunk when the caller leaves the choice
     // (chunk_planes = 0), by size class: the longest of whole columns, 96,
     // 64, 48, 32, 24, 16, 12 or 8 planes whose column diagonals hold on
-    // average at least 1.75 times the column waves the device keeps
+    // average at least 1.5 times the column waves the device keeps
     // resident.
     // Shorter chunks put more column-chunks on each diagonal (a sweep's
     // dependency chain is nbx + nby + nbz - 2 launches) at the price of
@@ -778,10 +778,10 @@ struct Engine3D
                 cus = v;
             }
         }
-        // resident column waves per CU: LDS rings of 22.8 KB (FP64) or
-        // 11.4 KB (4-byte elements) per wave
-        const int perCU = sweeps_f64() ? 7 : 14;
-        const double want = 1.75*cus*perCU;
+        // resident column waves per CU: LDS rings of 19.4 KB (FP64) or
+        // 9.7 KB (4-byte elements) per wave
+        const int perCU = sweeps_f64() ? 8 : 16;
+        const double want = 1.5*cus*perCU;
         for (int c : kChunkCandidates)
         {
             const int h = c == 0 ? maxNz : std::min(c, maxNz);

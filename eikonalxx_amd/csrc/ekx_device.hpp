@@ -17,10 +17,10 @@ constexpr int kColumnTile = 8;
 constexpr int kColumnHalo = kColumnTile + 1;          // 9: tile + upwind halo
 constexpr int kColumnPlane = kColumnHalo*kColumnHalo; // 81 values per plane
 constexpr int kColumnSkew = 2*(kColumnTile - 1);      // 14: lane a+b spread
-// LDS ring of planes per wave.  A plane is stored at step q, read by the
-// lanes through step q + 14 and written back at step q + 15, so 16 slots
-// are enough.
-constexpr int kColumnRing = 16;
+// LDS ring of planes per wave.  A plane enters at step q, is read by the
+// lanes through step q + 14 and is written back at the end of that step, so
+// 15 slots are enough.
+constexpr int kColumnRing = kColumnSkew + 1;
 
 __host__ __device__ inline int tiles_of(int n)
 {

@@ -262,12 +262,10 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // bulk instance that also runs the factored-box columns
     constexpr bool kMerged = !FACTORED && merged_factored<TS>();
     bool boxColumn = false;
-    constexpr int kSlot = kColumnPlane + 3;   // 84: pad the 81 values
-    // kColumnRing slots plus one spare that absorbs the stores of the
-    // padding steps after the last plane
+    constexpr int kSlot = kColumnPlane;       // 81 values per plane
     // per slot: the travel-time plane, then the slowness plane, so that
     // one address per lane serves both
-    __shared__ TS ring[WAVES][kColumnRing + 1][2][kSlot];
+    __shared__ TS ring[WAVES][kColumnRing][2][kSlot];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int itemIndex = blockIdx.x*WAVES + wave;
@@ -378,9 +376,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // point and serialise the prefetch.
     const int h = min(lane, kEdgeValues - 1);
     const int dstOwn = tileSlot(lane);
-    const int dstHalo = lane < kColumnHalo ? lidx(0, lane)
-                      : (lane < kEdgeValues ? lidx(lane - kColumnTile, 0)
-                                            : kColumnPlane);
+    const int dstHalo = h < kColumnHalo ? lidx(0, h)
+                                        : lidx(h - kColumnTile, 0);
     // Halo source: the upwind column's edge records (when that column ran
     // in this sweep: its last node X0 - 1 is at or past the octant start),
     // else the node itself in the tile records.  Per lane: the address at
@@ -497,31 +494,41 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     const int wbEdge = h < kColumnHalo ? lidx(kColumnTile, h)
                                        : lidx(h - kColumnTile, kColumnTile);
     const uint32_t eB = (uint32_t) h*sizeof(TS);
-    // Steps: every plane passes every lane and is written back; rounded up
-    // to whole groups of DL so the unrolled loop body has no exit (extra
-    // steps only repeat idempotent work: in-bounds loads, no computation,
-    // and a re-store of the last plane's final values).
-    const int nSteps = ((nPlanes + kColumnSkew + 1 + DL - 1)/DL)*DL;
+    // Steps: every plane passes every lane (plane q is finished at step
+    // q + 14) and is written back; rounded up to whole groups of DL so the
+    // unrolled loop body has no exit (extra steps only repeat idempotent
+    // work: in-bounds loads, no computation, and a re-store of the last
+    // plane's final values).
+    const int nSteps = ((nPlanes + kColumnSkew + DL - 1)/DL)*DL;
     TS(*myR)[2][kSlot] = ring[wave];
+    // Ring slots (planes mod 15): the entering plane's, the lane's plane
+    // q = step - (a + b), and the plane being written back.
+    int inSlot = 0;
+    int laneSlot = (kColumnRing - (a + b)) % kColumnRing;
+    int wbSlot = 0;
+    // the finished plane of the previous step, stored at the next one
+    TS pendT = TS(), pendE = TS();
+    char *pendTz = wbT, *pendEz = wbE;
+    auto nextSlot = [](int s) { return s == kColumnRing - 1 ? 0 : s + 1; };
     for (int t0 = 0; t0 < nSteps; t0 += DL)
     {
 #pragma unroll
         for (int u = 0; u < DL; ++u)
         {
             const int tc = t0 + u;
-            // (a) the plane prefetched DL steps ago enters the ring.  Branch
-            // free: past the last plane it goes to the spare slot, and the
-            // repeated loads write the slot's padding.
+            // (a) the plane prefetched DL steps ago enters the ring (past
+            // the last plane nothing enters: the slots still hold planes
+            // being finished and written back)
+            if (tc < nPlanes)
             {
-                const int slot = tc < nPlanes ? (tc & (kColumnRing - 1))
-                                              : kColumnRing;
-                TS *dT = myR[slot][0];
-                TS *dS = myR[slot][1];
+                TS *dT = myR[inSlot][0];
+                TS *dS = myR[inSlot][1];
                 dT[dstOwn] = bT0[u];
                 dS[dstOwn] = bS0[u];
                 dT[dstHalo] = bT1[u];
                 dS[dstHalo] = bS1[u];
             }
+            inSlot = nextSlot(inSlot);
             // (b) prefetch plane tc + DL into the freed registers (past the
             // last plane the load repeats it and is never stored)
             loadPlane(loadT, loadS, loadH, bT0[u], bT1[u], bS0[u], bS1[u]);
@@ -532,10 +539,17 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 loadH += haloStep;
                 if (P0 + loadQ <= ncz - 1) { loadS += stepS; }
             }
-            // (d, read) the plane the last lane finished a step ago; read
-            // with the neighbour values below, stored after the update
-            const TS wbValue = myR[wbQ & (kColumnRing - 1)][0][dstOwn];
-            const TS wbEdgeValue = myR[wbQ & (kColumnRing - 1)][0][wbEdge];
+            // (d, store) the plane finished in the previous step goes home
+            // with its edge records.  A store is issued at every step but
+            // the first (plane index clamped to the valid range; unchanged
+            // nodes and the tile's padding store their own values), so the
+            // prefetch loads have the same number of memory operations
+            // behind them.
+            if (tc > 0)
+            {
+                *(TS *) (pendTz + tB) = pendT;
+                *(TS *) (pendEz + eB) = pendE;
+            }
             // (c) every lane advances its node on plane tc - (a + b).  The
             // reads and the register shift run on every lane at every step
             // (outside the lane's planes they read unused ring slots and
@@ -544,8 +558,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // merging them at a branch.
             const int q = tc - (a + b);
             {
-                TS *sT = myR[q & (kColumnRing - 1)][0] + oA;
-                const TS *sS = myR[q & (kColumnRing - 1)][1] + oA;
+                TS *sT = myR[laneSlot][0] + oA;
+                const TS *sS = myR[laneSlot][1] + oA;
                 const TS ownCode = sT[kD];
                 R tOwn = St::time(ownCode, p);
                 const R t1 = St::time(sT[kC], p);
@@ -604,17 +618,18 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 ps01 = c01;
                 ps11 = c11;
             }
-            // (d, store) the finished plane goes home with its edge
-            // records.  The stores are issued at every step (plane index
-            // clamped to the ring's valid range; unchanged nodes and the
-            // tile's padding store their own values), so the prefetch loads
-            // above always have the same number of memory operations behind
-            // them.
-            *(TS *) (wbT + tB) = wbValue;
-            *(TS *) (wbE + eB) = wbEdgeValue;
-            if (tc >= kColumnSkew + 1 && wbQ < nPlanes - 1)
+            laneSlot = nextSlot(laneSlot);
+            // (d, read) plane clamp(tc - 14, 0, nPlanes - 1), which the
+            // last lane finished in this step: read after the updates
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+            pendT = myR[wbSlot][0][dstOwn];
+            pendE = myR[wbSlot][0][wbEdge];
+            pendTz = wbT;
+            pendEz = wbE;
+            if (tc >= kColumnSkew && wbQ < nPlanes - 1)
             {
                 wbQ += 1;
+                wbSlot = nextSlot(wbSlot);
                 wbT += stepT;
                 wbE += stepE;
             }
@@ -625,6 +640,9 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             __builtin_amdgcn_wave_barrier();
         }
     }
+    // the last finished plane
+    *(TS *) (pendTz + tB) = pendT;
+    *(TS *) (pendEz + eB) = pendE;
     // Convergence monitor: the wave's largest decrease, one atomic per
     // column.  Every lane of the wave is still active here (the early
     // returns above are whole-wave).
