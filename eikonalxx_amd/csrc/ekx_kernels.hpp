@@ -5,9 +5,9 @@
 namespace ekx
 {
 
-// Waves per workgroup of the column kernel.  Each wave owns 17 LDS planes
-// of 2 x 84 values (11.4 KB for 4-byte elements, 22.8 KB for FP64), so LDS
-// (160 KB per CU) bounds residency at 14 or 7 waves per CU; small blocks
+// Waves per workgroup of the column kernel.  Each wave owns 15 LDS slots
+// of 2 x 81 values (9.7 KB for 4-byte elements, 19.4 KB for FP64), so LDS
+// (160 KB per CU) bounds residency at 16 or 8 waves per CU; small blocks
 // let the dispatcher reach that bound.
 template <typename TS> constexpr int column_waves()
 {
@@ -15,9 +15,9 @@ template <typename TS> constexpr int column_waves()
 }
 
 // FP64 bulk launches also take the columns that overlap a factored box
-// (full local update in those waves): LDS already holds FP64 at 7 resident
-// waves per CU, below what the larger register count of the factored
-// branch allows, so one launch per diagonal costs no occupancy.  The
+// (full local update in those waves): LDS holds FP64 at 8 resident waves
+// per CU, what the register count of the factored branch (2 waves per
+// SIMD) allows too, so one launch per diagonal costs no occupancy.  The
 // 4-byte types keep those columns in the separate FACTORED instance.
 template <typename TS> constexpr bool merged_factored()
 {
