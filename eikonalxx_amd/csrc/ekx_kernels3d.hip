@@ -251,6 +251,16 @@ sweep_hyperplane3d(const Problem3D *__restrict__ problems, int problem,
 // item list, for the 4-byte types, whose bulk instance keeps the register
 // count of the Cartesian update.
 // ---------------------------------------------------------------------------
+// Keeps the compiler from moving memory accesses across this point.  The
+// column kernel's lanes exchange values through LDS within one wave, whose
+// DS instructions execute in issue order, so program order is all the
+// exchange needs: no wait for the LDS counter (a wave-scope fence made the
+// compiler wait for every outstanding LDS access before the next store).
+__device__ __forceinline__ void lds_order()
+{
+    __asm__ __volatile__("" ::: "memory");
+}
+
 template <typename TS, int WAVES, int DL, bool FACTORED>
 __global__ void __launch_bounds__(64*WAVES)
 sweep_columns3d(const Problem3D *__restrict__ problems,
@@ -506,8 +516,33 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     int inSlot = 0;
     int laneSlot = (kColumnRing - (a + b)) % kColumnRing;
     int wbSlot = 0;
-    // the finished plane of the previous step, stored at the next one
-    TS pendT = TS(), pendE = TS();
+    const int lastSlot = (nPlanes - 1) % kColumnRing;
+    // The finished plane of the previous step, stored at the next one.
+    // Every store writes final values only (columns of the same launch may
+    // read a tile's plane P0 edge records meanwhile), so step 0 stores
+    // plane 0 as loaded: the lane's own record element, and for the edge
+    // record the value loaded by the lane whose ring position is wbEdge
+    // (lane 16's halo value for h = 0).
+    TS pendT = bT0[0];
+    TS pendE;
+    {
+        int xe, ye;
+        if (h < kColumnHalo)
+        {
+            xe = negX ? 0 : kColumnTile - 1;
+            ye = negY ? kColumnTile - h : h - 1;
+        }
+        else
+        {
+            const int c = h - kColumnTile;
+            xe = negX ? kColumnTile - c : c - 1;
+            ye = negY ? 0 : kColumnTile - 1;
+        }
+        const int src = h == 0 ? 0 : xe + kColumnTile*ye;
+        const TS fromTile = __shfl(bT0[0], src, 64);
+        const TS fromHalo = __shfl(bT1[0], kEdgeValues - 1, 64);
+        pendE = h == 0 ? fromHalo : fromTile;
+    }
     char *pendTz = wbT, *pendEz = wbE;
     auto nextSlot = [](int s) { return s == kColumnRing - 1 ? 0 : s + 1; };
     for (int t0 = 0; t0 < nSteps; t0 += DL)
@@ -516,13 +551,16 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         for (int u = 0; u < DL; ++u)
         {
             const int tc = t0 + u;
-            // (a) the plane prefetched DL steps ago enters the ring (past
-            // the last plane nothing enters: the slots still hold planes
-            // being finished and written back)
-            if (tc < nPlanes)
+            // (a) the plane prefetched DL steps ago enters the ring.  Past
+            // the last plane the repeated loads go to the slot of a plane
+            // already written back, never to the last plane's slot, which
+            // the clamped write-back keeps re-reading; branch free, so the
+            // LDS and memory counters stay exact.
             {
-                TS *dT = myR[inSlot][0];
-                TS *dS = myR[inSlot][1];
+                const int slot = (tc >= nPlanes && inSlot == lastSlot)
+                                     ? nextSlot(inSlot) : inSlot;
+                TS *dT = myR[slot][0];
+                TS *dS = myR[slot][1];
                 dT[dstOwn] = bT0[u];
                 dS[dstOwn] = bS0[u];
                 dT[dstHalo] = bT1[u];
@@ -540,16 +578,13 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 if (P0 + loadQ <= ncz - 1) { loadS += stepS; }
             }
             // (d, store) the plane finished in the previous step goes home
-            // with its edge records.  A store is issued at every step but
-            // the first (plane index clamped to the valid range; unchanged
-            // nodes and the tile's padding store their own values), so the
-            // prefetch loads have the same number of memory operations
+            // with its edge records.  A store is issued at every step
+            // (plane index clamped to the valid range; unchanged nodes and
+            // the tile's padding store their own values), so the prefetch
+            // loads always have the same number of memory operations
             // behind them.
-            if (tc > 0)
-            {
-                *(TS *) (pendTz + tB) = pendT;
-                *(TS *) (pendEz + eB) = pendE;
-            }
+            *(TS *) (pendTz + tB) = pendT;
+            *(TS *) (pendEz + eB) = pendE;
             // (c) every lane advances its node on plane tc - (a + b).  The
             // reads and the register shift run on every lane at every step
             // (outside the lane's planes they read unused ring slots and
@@ -621,7 +656,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             laneSlot = nextSlot(laneSlot);
             // (d, read) plane clamp(tc - 14, 0, nPlanes - 1), which the
             // last lane finished in this step: read after the updates
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+            lds_order();
             pendT = myR[wbSlot][0][dstOwn];
             pendE = myR[wbSlot][0][wbEdge];
             pendTz = wbT;
@@ -634,9 +669,8 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
                 wbE += stepE;
             }
             // LDS traffic of this step must not be reordered with the next
-            // step's: other lanes of the wave consume it.  The fence covers
-            // LDS only, so global loads stay free to move.
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront", "local");
+            // step's: other lanes of the wave consume it.
+            lds_order();
             __builtin_amdgcn_wave_barrier();
         }
     }
