@@ -235,6 +235,7 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                              chains=args.chains,
                              prefetch_planes=args.prefetch,
                              chunk_planes=args.chunk,
+                             launch_order=args.launch_order,
                              use_graph=not args.no_graph)
     batch = ekx.Batch3D()
     batch.initialize(opts, [(geom, slow, s) for s in w["sources"]])
@@ -292,7 +293,8 @@ def run_mode(ekx, mode, w, slow, geom, picks, ref_fields, args):
                                   kernel_timing=True,
                                   chains=args.timing_chains,
                                   prefetch_planes=args.prefetch,
-                                  chunk_planes=args.chunk)
+                                  chunk_planes=args.chunk,
+                                  launch_order=args.launch_order)
         tb = ekx.Batch3D()
         tb.initialize(topts, [(geom, slow, s) for s in w["sources"]])
         tb.solve()
@@ -389,6 +391,10 @@ def main():
     ap.add_argument("--chains", type=int, default=0,
                     help="independent launch chains (0: the library's "
                          "choice)")
+    ap.add_argument("--launch-order", type=int, default=0,
+                    help="FP64 column sweeps: 1 one launch per diagonal, "
+                         "2 one work-queue launch per sweep (0: the "
+                         "library's choice)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-iterations", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")

@@ -23,7 +23,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.environ.get("EKX_LIBRARY_PATH",
                               os.path.join(_HERE, "libeikonalxx_amd.so"))
 
-__all__ = ["SolverAlgorithm", "Precision", "Schedule", "Kernel", "Ordering",
+__all__ = ["SolverAlgorithm", "Precision", "Schedule", "Kernel",
+           "LaunchOrder", "Ordering",
            "SolverOptions", "Geometry3D", "Geometry2D", "Solver3D",
            "Batch3D", "Solver2D", "SolveStats", "library", "device_count",
            "synchronize", "DeviceError"]
@@ -55,6 +56,12 @@ class Kernel(enum.IntEnum):
     COLUMN = 2
 
 
+class LaunchOrder(enum.IntEnum):
+    AUTO = 0
+    DIAGONALS = 1
+    QUEUE = 2
+
+
 class Ordering(enum.IntEnum):
     """include/eikonalxx/enums.hpp:7-40"""
     Natural = 0
@@ -79,7 +86,8 @@ class _Options(ctypes.Structure):
                 ("kernel_timing", ctypes.c_int),
                 ("chains", ctypes.c_int),
                 ("prefetch_planes", ctypes.c_int),
-                ("chunk_planes", ctypes.c_int)]
+                ("chunk_planes", ctypes.c_int),
+                ("launch_order", ctypes.c_int)]
 
 
 class _Geometry3D(ctypes.Structure):
@@ -313,7 +321,8 @@ class SolverOptions:
                  schedule=Schedule.COMPLETED, kernel=Kernel.AUTO,
                  convergence_tolerance=None, early_exit=False,
                  use_graph=True, verbosity=0, kernel_timing=False,
-                 chains=0, prefetch_planes=0, chunk_planes=0):
+                 chains=0, prefetch_planes=0, chunk_planes=0,
+                 launch_order=LaunchOrder.AUTO):
         o = _Options()
         library().ekx_solver_options_default(ctypes.byref(o))
         if number_of_sweeps is not None:
@@ -333,6 +342,7 @@ class SolverOptions:
         o.chains = int(chains)
         o.prefetch_planes = int(prefetch_planes)
         o.chunk_planes = int(chunk_planes)
+        o.launch_order = int(launch_order)
         o.verbosity = int(verbosity)
         self._o = o
 

@@ -165,6 +165,10 @@ ekx_status check_options(const ekx_solver_options *o)
     {
         return fail(EKX_EINVAL, "chunk_planes must be >= 0");
     }
+    if (o->launch_order < EKX_LAUNCH_AUTO || o->launch_order > EKX_LAUNCH_QUEUE)
+    {
+        return fail(EKX_EINVAL, "launch_order must be 0, 1 or 2");
+    }
     if (o->prefetch_planes != 0 && o->prefetch_planes != 4 &&
         o->prefetch_planes != 8 && o->prefetch_planes != 12)
     {
@@ -384,6 +388,16 @@ struct Engine3D
     // at most ceil((2r+1)/8)+1 columns per axis, and each (bx, by) meets a
     // diagonal at one bz at most.
     int boxPerDiag = 0;
+    // Work-queue sweeps (launch order QUEUE): per problem the first
+    // completion flag of its column chunks (flag of chunk (bx, by, bz):
+    // base + (bz*nTY + by)*nTX + bx); queueWords holds the flags, one item
+    // counter per chain and sweep of a sequence, then the error word.
+    DeviceBuffer<int> dFlagBase;
+    DeviceBuffer<unsigned int> queueWords;
+    int64_t queueFlags = 0;
+    int queueMaxSweeps = 0;
+    int queueWaves = 0;          // column waves the device keeps resident
+    uint64_t queueWaitTicks = 0; // bound of one dependency wait
     hipStream_t stream = nullptr;
     hipStream_t sideStream = nullptr;   // factored-box columns
     hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -489,6 +503,13 @@ struct Engine3D
     int kernel() const
     {
         return opt.kernel == EKX_KERNEL_AUTO ? EKX_KERNEL_COLUMN : opt.kernel;
+    }
+    // Sweeps of FP64 fields by the column kernel in work-queue form (the
+    // 4-byte types keep their factored-box columns in separate launches).
+    template <typename TS> bool queued() const
+    {
+        return kernel() == EKX_KERNEL_COLUMN && merged_factored<TS>() &&
+               opt.launch_order == EKX_LAUNCH_QUEUE;
     }
 
     // Allocates buffers and the column work lists for the given geometries;
@@ -751,7 +772,89 @@ struct Engine3D
         EKX_HIP(hipMemcpyAsync(dItems.ptr, flat.data(),
                                flat.size()*sizeof(ColumnItem),
                                hipMemcpyHostToDevice, stream));
+        // work-queue flags: one per column chunk of every problem
+        std::vector<int> flagBase(n);
+        int64_t nFlags = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            const ekx_geometry3d &g = geoms[i];
+            flagBase[i] = (int) nFlags;
+            nFlags += (int64_t) ((g.nx + kColumnTile - 1)/kColumnTile)*
+                      ((g.ny + kColumnTile - 1)/kColumnTile)*
+                      ((g.nz + chunkH - 1)/chunkH);
+        }
+        if (nFlags > INT32_MAX)
+        {
+            return fail(EKX_EINVAL, "too many column chunks for one batch");
+        }
+        queueFlags = nFlags;
+        queueMaxSweeps = 8*(std::max(o.number_of_sweeps, 1) + 1);
+        EKX_CHECK(dFlagBase.allocate(flagBase.size()));
+        EKX_HIP(hipMemcpyAsync(dFlagBase.ptr, flagBase.data(),
+                               flagBase.size()*sizeof(int),
+                               hipMemcpyHostToDevice, stream));
+        const size_t words = (size_t) queueFlags +
+                             (size_t) nChains*queueMaxSweeps + 1;
+        EKX_CHECK(queueWords.allocate(words));
+        EKX_HIP(hipMemsetAsync(queueWords.ptr, 0, words*sizeof(unsigned int),
+                               stream));
+        {
+            int device = 0, cus = 256, v = 0;
+            if (hipGetDevice(&device) == hipSuccess &&
+                hipDeviceGetAttribute(&v,
+                                      hipDeviceAttributeMultiprocessorCount,
+                                      device) == hipSuccess && v > 0)
+            {
+                cus = v;
+            }
+            // FP64 rings: 19.4 KB of LDS per wave, 8 waves per CU
+            queueWaves = 8*cus;
+            // 250 ms of the device's wall clock (rate in kHz)
+            int kHz = 0;
+            if (hipDeviceGetAttribute(&kHz, hipDeviceAttributeWallClockRate,
+                                      device) != hipSuccess || kHz <= 0)
+            {
+                kHz = 100000;
+            }
+            queueWaitTicks = (uint64_t) kHz*250;
+        }
         return EKX_OK;
+    }
+
+    // The error word of the work queue: set when a dependency wait of a
+    // sweep exceeded its bound (the launch then finished with unordered
+    // chunks).  Read after a synchronised solve, then cleared.
+    ekx_status check_queue_error()
+    {
+        if (!queued<double>() || queueWords.ptr == nullptr) { return EKX_OK; }
+        unsigned int err = 0;
+        unsigned int *w = queueWords.ptr + queueFlags +
+                          (size_t) nChains*queueMaxSweeps;
+        EKX_HIP(hipMemcpy(&err, w, sizeof(err), hipMemcpyDeviceToHost));
+        if (err != 0)
+        {
+            EKX_HIP(hipMemset(w, 0, sizeof(err)));
+            return fail(EKX_EHIP, "column work queue: a dependency wait "
+                                  "exceeded its bound");
+        }
+        return EKX_OK;
+    }
+
+    // The work queue of chain c's sweep number `epoch` (1-based) of the
+    // current sequence.
+    ColumnQueue column_queue(int c, unsigned int epoch) const
+    {
+        ColumnQueue q;
+        q.flagBase = dFlagBase.ptr;
+        q.flags = queueWords.ptr;
+        q.counter = queueWords.ptr + queueFlags +
+                    (size_t) c*queueMaxSweeps + (epoch - 1);
+        q.error = queueWords.ptr + queueFlags +
+                  (size_t) nChains*queueMaxSweeps;
+        q.epoch = epoch;
+        q.pad = 0;
+        q.waitTicks = queueWaitTicks;
+        return q;
     }
 
     // Planes per z-chunk when the caller leaves the choice
@@ -979,12 +1082,12 @@ struct Engine3D
 
     template <typename TS>
     ekx_status enqueue_sweep(int sweep, bool initialize, TS *T, const TS *S,
-                             ekx_solve_stats &st)
+                             unsigned int epoch, ekx_solve_stats &st)
     {
         count_sweep<TS>(sweep, initialize, st);
         int64_t launches = 0;
         ekx_status rc = enqueue_sweep_launches<TS>(sweep, initialize, T, S,
-                                                   launches);
+                                                   epoch, launches);
         st.kernel_launches += launches;
         if (sizeof(TS) == 8) { st.kernel_launches_fp64 += launches; }
         return rc;
@@ -1003,13 +1106,19 @@ struct Engine3D
     // chain's problems.
     template <typename TS>
     ekx_status enqueue_chain_sweep(int c, int sweep, bool initialize, TS *T,
-                                   int64_t &launches)
+                                   unsigned int epoch, int64_t &launches)
     {
         hipStream_t cs = chainStreams[c];
         const TS *SB = bricks<TS>();
         const int p0 = chainBegin[c];
         const int np = chainBegin[c + 1] - p0;
         const int *ds = diagStart.data() + (size_t) c*(nDiag + 1);
+        if (queued<TS>())
+        {
+            EKX_CHECK(enqueue_queued_sweep<TS>(c, sweep, initialize, T,
+                                               epoch, cs, launches));
+            return EKX_OK;
+        }
         for (int d = 0; d < nDiag; ++d)
         {
             const int count = ds[d + 1] - ds[d];
@@ -1040,11 +1149,42 @@ struct Engine3D
         return EKX_OK;
     }
 
+    // One sweep of chain c in work-queue form: one launch over the chain's
+    // items, all diagonals.
+    template <typename TS>
+    ekx_status enqueue_queued_sweep(int c, int sweep, bool initialize, TS *T,
+                                    unsigned int epoch, hipStream_t s,
+                                    int64_t &launches)
+    {
+        if (epoch < 1 || (int) epoch > queueMaxSweeps)
+        {
+            return fail(EKX_EINVAL, "work queue: too many sweeps in one "
+                                    "sequence");
+        }
+        const int *ds = diagStart.data() + (size_t) c*(nDiag + 1);
+        const int count = ds[nDiag] - ds[0];
+        if (count <= 0) { return EKX_OK; }
+        if (timed_sweep<TS>(initialize)) { EKX_CHECK(record_timing_event(s)); }
+        EKX_HIP(launch_queued_columns3d<TS>(
+            dProblems.ptr, dItems.ptr + ds[0], count, sweep,
+            initialize ? 1 : 0, chunkH, T, bricks<TS>(), change.ptr,
+            column_queue(c, epoch), queueWaves, depth(), s));
+        if (timed_sweep<TS>(initialize)) { EKX_CHECK(record_timing_event(s)); }
+        launches += 1;
+        return EKX_OK;
+    }
+
     template <typename TS>
     ekx_status enqueue_sweep_launches(int sweep, bool initialize, TS *T,
-                                      const TS *S, int64_t &launches)
+                                      const TS *S, unsigned int epoch,
+                                      int64_t &launches)
     {
         const int n = (int) problems.size();
+        if (kernel() == EKX_KERNEL_COLUMN && queued<TS>())
+        {
+            return enqueue_queued_sweep<TS>(0, sweep, initialize, T, epoch,
+                                            stream, launches);
+        }
         if (kernel() == EKX_KERNEL_HYPERPLANE)
         {
             for (int i = 0; i < n; ++i)
@@ -1147,11 +1287,22 @@ struct Engine3D
         {
             for (int s = 0; s < 8; ++s) { list.push_back({s, false}); }
         }
+        if (queued<TS>())
+        {
+            // a new sequence of work-queue sweeps: flags and counters back
+            // to zero, epochs count from 1
+            EKX_HIP(launch_clear_words(queueWords.ptr,
+                                       queueFlags + (int64_t) nChains*
+                                                        queueMaxSweeps,
+                                       stream));
+        }
         if (nChains == 1)
         {
+            unsigned int epoch = 0;
             for (const auto &sw : list)
             {
-                EKX_CHECK(enqueue_sweep<TS>(sw.first, sw.second, T, S, st));
+                EKX_CHECK(enqueue_sweep<TS>(sw.first, sw.second, T, S,
+                                            ++epoch, st));
             }
         }
         else
@@ -1167,10 +1318,11 @@ struct Engine3D
             for (int c = 0; c < nChains; ++c)
             {
                 EKX_HIP(hipStreamWaitEvent(chainStreams[c], evFork, 0));
+                unsigned int epoch = 0;
                 for (const auto &sw : list)
                 {
                     EKX_CHECK(enqueue_chain_sweep<TS>(c, sw.first, sw.second,
-                                                      T, launches));
+                                                      T, ++epoch, launches));
                 }
                 EKX_HIP(hipEventRecord(chainEvents[c], chainStreams[c]));
             }
@@ -1291,6 +1443,7 @@ struct Engine3D
             EKX_CHECK(run_segment<TS>(0, T, S, true, 1, st));
             EKX_HIP(hipEventRecord(evStop, stream));
             EKX_HIP(hipEventSynchronize(evStop));
+            EKX_CHECK(check_queue_error());
         }
         EKX_HIP(hipEventElapsedTime(&ms, evStart, evStop));
         return EKX_OK;
@@ -1468,6 +1621,7 @@ struct Engine3D
         EKX_HIP(hipEventRecord(evStop, stream));
         EKX_HIP(hipEventSynchronize(evStop));
         EKX_HIP(hipGetLastError());
+        EKX_CHECK(check_queue_error());
         float ms = 0;
         EKX_HIP(hipEventElapsedTime(&ms, evStart, evStop));
         st.last_solve_ms = ms;
@@ -1876,6 +2030,7 @@ void ekx_solver_options_default(ekx_solver_options *o)
     o->chains = 0;
     o->prefetch_planes = 0;
     o->chunk_planes = 0;
+    o->launch_order = EKX_LAUNCH_AUTO;
 }
 
 ekx_status ekx_device_count(int *count)

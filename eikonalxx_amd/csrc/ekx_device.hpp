@@ -114,6 +114,23 @@ struct ColumnItem
     int bx, by, bz;             // column and chunk index, permuted frame
 };
 
+// Work queue of one sweep launch in work-queue form: item i of problem p
+// publishes the sweep's epoch in flags[flagBase[p] + (bz*nTY + by)*nTX + bx]
+// once finished; counter hands out the launch's items in list order; error
+// is set when a wait exceeded waitTicks of the device's wall clock (then
+// later waits return at once).  Flags and counters are zeroed at the start
+// of each sequence of sweeps, whose epochs count from 1.
+struct ColumnQueue
+{
+    const int *flagBase;
+    unsigned int *flags;
+    unsigned int *counter;
+    unsigned int *error;
+    unsigned int epoch;
+    unsigned int pad;
+    uint64_t waitTicks;
+};
+
 // A receiver located in one problem's grid (Station3D, src/station3d.cpp:
 // 158-291): offset from the model origin and the cell that holds it.
 struct StationLoc3D

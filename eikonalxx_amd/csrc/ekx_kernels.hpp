@@ -57,6 +57,21 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      TS *T, const TS *S,
                                      unsigned int *change,
                                      hipStream_t stream);
+// A whole sweep of one chain's items (the diagonal launches' lists,
+// concatenated in diagonal order) in work-queue form: `waves` resident
+// waves take the items in order and start each once its upwind chunks have
+// published completion in q (the factored-box chunks included; FP64
+// fields only).
+template <typename TS>
+hipError_t launch_queued_columns3d(const Problem3D *dProblems,
+                                   const ColumnItem *dItems, int nItems,
+                                   int sweep, int initialize, int chunk, TS *T,
+                                   const TS *S, unsigned int *change,
+                                   const ColumnQueue &q, int waves, int depth,
+                                   hipStream_t stream);
+// Zeroes n words of device memory (a kernel, so that captured graphs hold
+// no memset node).
+hipError_t launch_clear_words(unsigned int *w, int64_t n, hipStream_t stream);
 // Tiled layout: the slowness bricks of one model (natural cells s of an
 // nx x ny x nz node grid) into b, and a natural-order copy (nx*ny*nz
 // values) of problem p's tiled field (T: the whole buffer).

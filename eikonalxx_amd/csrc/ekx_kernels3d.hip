@@ -261,48 +261,58 @@ __device__ __forceinline__ void lds_order()
     __asm__ __volatile__("" ::: "memory");
 }
 
-template <typename TS, int WAVES, int DL, bool FACTORED>
-__global__ void __launch_bounds__(64*WAVES)
-sweep_columns3d(const Problem3D *__restrict__ problems,
-                const ColumnItem *__restrict__ items, int nItems, int sweep,
-                int initialize, int diagonal, int boxPerDiag, int chunk,
-                TS *__restrict__ T, const TS *__restrict__ S,
-                unsigned int *__restrict__ change)
+// Travel-time accesses of a column.  COHERENT (work-queue launches, where
+// chunks of one launch on different XCDs exchange planes): agent-scope
+// relaxed atomics, i.e. loads and stores that bypass the XCD-local caches
+// the other XCDs do not see.
+template <bool COHERENT, typename TS>
+__device__ __forceinline__ TS load_time(const TS *p)
+{
+    if constexpr (COHERENT)
+    {
+        using U = typename std::conditional<sizeof(TS) == 8, uint64_t,
+                                            uint32_t>::type;
+        const U u = __hip_atomic_load(const_cast<U *>(
+                                          reinterpret_cast<const U *>(p)),
+                                      __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        return __builtin_bit_cast(TS, u);
+    }
+    else
+    {
+        return *p;
+    }
+}
+
+template <bool COHERENT, typename TS>
+__device__ __forceinline__ void store_time(TS *p, TS v)
+{
+    if constexpr (COHERENT)
+    {
+        using U = typename std::conditional<sizeof(TS) == 8, uint64_t,
+                                            uint32_t>::type;
+        __hip_atomic_store(reinterpret_cast<U *>(p),
+                           __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    else
+    {
+        *p = v;
+    }
+}
+
+// One column chunk (problem pIndex, column (bx, by), chunk bz) of one sweep
+// on one wave; myR is the wave's LDS ring.
+template <typename TS, int DL, bool FACTORED, bool COHERENT = false>
+__device__ __forceinline__ void run_column(
+    const Problem3D *__restrict__ problems, int pIndex, int bx, int by,
+    int bz, int sweep, int initialize, int chunk, TS *__restrict__ T,
+    const TS *__restrict__ S, unsigned int *__restrict__ change,
+    TS (*myR)[2][kColumnPlane], int lane)
 {
     // bulk instance that also runs the factored-box columns
     constexpr bool kMerged = !FACTORED && merged_factored<TS>();
     bool boxColumn = false;
-    constexpr int kSlot = kColumnPlane;       // 81 values per plane
-    // per slot: the travel-time plane, then the slowness plane, so that
-    // one address per lane serves both
-    __shared__ TS ring[WAVES][kColumnRing][2][kSlot];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int itemIndex = blockIdx.x*WAVES + wave;
-    if (itemIndex >= nItems) { return; }
-    int pIndex, bx, by, bz;
-    if (FACTORED)
-    {
-        // item = (problem, k): the k-th (bx, by) of the box, at the z-chunk
-        // that puts it on this diagonal bx + by + bz
-        pIndex = itemIndex/boxPerDiag;
-        const int k = itemIndex - pIndex*boxPerDiag;
-        const BoxColumns bc = box_columns(problems[pIndex], sweep, chunk);
-        const int nbx = bc.bxHi - bc.bxLo + 1, nby = bc.byHi - bc.byLo + 1;
-        if (nbx <= 0 || nby <= 0 || k >= nbx*nby) { return; }
-        bx = bc.bxLo + k%nbx;
-        by = bc.byLo + k/nbx;
-        bz = diagonal - bx - by;
-        if (bz < bc.bzLo || bz > bc.bzHi) { return; }
-    }
-    else
-    {
-        const ColumnItem item = items[itemIndex];
-        pIndex = item.problem;
-        bx = item.bx;
-        by = item.by;
-        bz = item.bz;
-    }
     pIndex = __builtin_amdgcn_readfirstlane(pIndex);
     const Problem3D &p = problems[pIndex];
     if (!FACTORED)
@@ -453,12 +463,16 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     {
         return *(const TS *) (base + off);
     };
+    // COHERENT: the halo (the upwind chunks' edge records) always; of the
+    // chunk's own tile records only the first plane, which the previous
+    // chunk of the column wrote (its other planes were last written by an
+    // earlier launch)
     auto loadPlane = [&](const char *Tz, const char *Sz, const char *Hz,
                          TS &vT0, TS &vT1, TS &vS0, TS &vS1)
     {
         vT0 = at(Tz, tB);
         vS0 = at(Sz, sB0);
-        vT1 = *(const TS *) Hz;
+        vT1 = load_time<COHERENT>((const TS *) Hz);
         vS1 = at(Sz, sB1);
     };
 
@@ -472,6 +486,10 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         const int q = min(u, nPlanes - 1);
         loadPlane(planeT(q), planeS(q), haloP0 + (int64_t) q*haloStep,
                   bT0[u], bT1[u], bS0[u], bS1[u]);
+        if (COHERENT && u == 0)
+        {
+            bT0[0] = load_time<true>((const TS *) (planeT(0) + tB));
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     // Running plane pointers: the next prefetch (plane min(tc + DL,
@@ -510,7 +528,6 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
     // work: in-bounds loads, no computation, and a re-store of the last
     // plane's final values).
     const int nSteps = ((nPlanes + kColumnSkew + DL - 1)/DL)*DL;
-    TS(*myR)[2][kSlot] = ring[wave];
     // Ring slots (planes mod 15): the entering plane's, the lane's plane
     // q = step - (a + b), and the plane being written back.
     int inSlot = 0;
@@ -584,7 +601,7 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             // loads always have the same number of memory operations
             // behind them.
             *(TS *) (pendTz + tB) = pendT;
-            *(TS *) (pendEz + eB) = pendE;
+            store_time<COHERENT>((TS *) (pendEz + eB), pendE);
             // (c) every lane advances its node on plane tc - (a + b).  The
             // reads and the register shift run on every lane at every step
             // (outside the lane's planes they read unused ring slots and
@@ -674,9 +691,11 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
             __builtin_amdgcn_wave_barrier();
         }
     }
-    // the last finished plane
-    *(TS *) (pendTz + tB) = pendT;
-    *(TS *) (pendEz + eB) = pendE;
+    // the last finished plane (COHERENT: the next chunk of the column reads
+    // it as its first plane; the earlier stores of the same plane wrote the
+    // same values)
+    store_time<COHERENT>((TS *) (pendTz + tB), pendT);
+    store_time<COHERENT>((TS *) (pendEz + eB), pendE);
     // Convergence monitor: the wave's largest decrease, one atomic per
     // column.  Every lane of the wave is still active here (the early
     // returns above are whole-wave).
@@ -692,6 +711,181 @@ sweep_columns3d(const Problem3D *__restrict__ problems,
         {
             atomicMax(change, __float_as_uint(d));
         }
+    }
+}
+
+template <typename TS, int WAVES, int DL, bool FACTORED>
+__global__ void __launch_bounds__(64*WAVES)
+sweep_columns3d(const Problem3D *__restrict__ problems,
+                const ColumnItem *__restrict__ items, int nItems, int sweep,
+                int initialize, int diagonal, int boxPerDiag, int chunk,
+                TS *__restrict__ T, const TS *__restrict__ S,
+                unsigned int *__restrict__ change)
+{
+    constexpr int kSlot = kColumnPlane;       // 81 values per plane
+    // per slot: the travel-time plane, then the slowness plane, so that
+    // one address per lane serves both
+    __shared__ TS ring[WAVES][kColumnRing][2][kSlot];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int itemIndex = blockIdx.x*WAVES + wave;
+    if (itemIndex >= nItems) { return; }
+    int pIndex, bx, by, bz;
+    if (FACTORED)
+    {
+        // item = (problem, k): the k-th (bx, by) of the box, at the z-chunk
+        // that puts it on this diagonal bx + by + bz
+        pIndex = itemIndex/boxPerDiag;
+        const int k = itemIndex - pIndex*boxPerDiag;
+        const BoxColumns bc = box_columns(problems[pIndex], sweep, chunk);
+        const int nbx = bc.bxHi - bc.bxLo + 1, nby = bc.byHi - bc.byLo + 1;
+        if (nbx <= 0 || nby <= 0 || k >= nbx*nby) { return; }
+        bx = bc.bxLo + k%nbx;
+        by = bc.byLo + k/nbx;
+        bz = diagonal - bx - by;
+        if (bz < bc.bzLo || bz > bc.bzHi) { return; }
+    }
+    else
+    {
+        const ColumnItem item = items[itemIndex];
+        pIndex = item.problem;
+        bx = item.bx;
+        by = item.by;
+        bz = item.bz;
+    }
+    run_column<TS, DL, FACTORED>(problems, pIndex, bx, by, bz, sweep,
+                                 initialize, chunk, T, S, change, ring[wave],
+                                 lane);
+}
+
+// ---------------------------------------------------------------------------
+// Work-queue form of a sweep: one launch per sweep (and chain) whose waves
+// take column chunks from a counter in the order of the item list (by
+// diagonal bx + by + bz) and start each once its upwind chunks
+// (bx - 1, by, bz), (bx, by - 1, bz) and (bx, by, bz - 1) have published
+// their completion, instead of one launch per diagonal with a barrier
+// between diagonals.  A chunk's halo comes from the edge records of its x
+// and y upwind chunks and its first plane from its z upwind chunk (the
+// diagonal launches order exactly these), so the nodes see the same values
+// and the field is the same.  Every item waited for was taken earlier by a
+// running wave, so the waits end; each is still bounded in time and, past
+// the bound, flags an error and goes on (later waits then return at once),
+// so that every wave exits.
+// Waves of different XCDs exchange planes through device memory: the
+// planes that cross chunks (edge records, the first plane of a chunk, the
+// last plane it hands to the next chunk) are read and written with
+// agent-scope atomics, which bypass the XCD-local caches, and a chunk
+// publishes only after all its stores have completed.  (Agent-scope
+// release and acquire fences, which write back and invalidate a whole L2,
+// made the sweeps 1.7 times slower.)
+// ---------------------------------------------------------------------------
+// Every lane runs the queue's control flow (the same address and value in
+// all lanes, the loop exits on readfirstlane of the loaded word): a branch
+// taken by lane 0 alone at the end of the loop body let the compiler keep
+// the other lanes iterating without it, where readfirstlane then read
+// theirs.
+// A wait is bounded in time (waitTicks of the wall clock, far above the
+// longest chain of chunks a sweep can hold up), not in polls: a sweep whose
+// last diagonals hold few chunks leaves waves waiting many column times.
+__device__ __forceinline__ void wait_published(const ColumnQueue &q, int f)
+{
+    if (f < 0) { return; }
+    uint64_t t0 = 0;
+    for (unsigned int spins = 0;; ++spins)
+    {
+        const unsigned int v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(q.flags + f, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT));
+        if (v >= q.epoch) { return; }
+        if ((spins & 63) == 63)
+        {
+            // the clock starts after the first 64 polls
+            const uint64_t now = (uint64_t) wall_clock64();
+            if (spins == 63) { t0 = now; }
+            const bool failed = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(q.error, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT)) != 0;
+            const bool late = now - t0 > q.waitTicks;
+            if (failed || late)
+            {
+                __hip_atomic_fetch_or(q.error, 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+template <typename TS, int DL>
+__global__ void __launch_bounds__(64)
+sweep_columns3d_queued(const Problem3D *__restrict__ problems,
+                       const ColumnItem *__restrict__ items, int nItems,
+                       int sweep, int initialize, int chunk,
+                       TS *__restrict__ T, const TS *__restrict__ S,
+                       unsigned int *__restrict__ change, ColumnQueue q)
+{
+    __shared__ TS ring[kColumnRing][2][kColumnPlane];
+    const int lane = threadIdx.x & 63;
+    for (;;)
+    {
+        // one item per wave: lane 0 adds 1, the others 0, and every lane
+        // reads lane 0's value
+        const unsigned int taken = __hip_atomic_fetch_add(
+            q.counter, lane == 0 ? 1u : 0u, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+        const int index = (int) __builtin_amdgcn_readfirstlane(taken);
+        if (index >= nItems) { break; }
+        const ColumnItem item = items[index];
+        const int pIndex = __builtin_amdgcn_readfirstlane(item.problem);
+        const int bx = __builtin_amdgcn_readfirstlane(item.bx);
+        const int by = __builtin_amdgcn_readfirstlane(item.by);
+        const int bz = __builtin_amdgcn_readfirstlane(item.bz);
+        const Problem3D &p = problems[pIndex];
+        const int nbx = p.nTX, nby = p.nTY;
+        const int self = q.flagBase[pIndex] + (bz*nby + by)*nbx + bx;
+        // the three flags in one round trip; a wait only for those not
+        // yet published
+        const int fx = bx > 0 ? self - 1 : self;
+        const int fy = by > 0 ? self - nbx : self;
+        const int fz = bz > 0 ? self - nbx*nby : self;
+        const unsigned int vx = __hip_atomic_load(
+            q.flags + fx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int vy = __hip_atomic_load(
+            q.flags + fy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int vz = __hip_atomic_load(
+            q.flags + fz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ready = __builtin_amdgcn_readfirstlane(
+            (bx == 0 || vx >= q.epoch) && (by == 0 || vy >= q.epoch) &&
+            (bz == 0 || vz >= q.epoch) ? 1 : 0) != 0;
+        if (!ready)
+        {
+            wait_published(q, bx > 0 ? fx : -1);
+            wait_published(q, by > 0 ? fy : -1);
+            wait_published(q, bz > 0 ? fz : -1);
+        }
+        // the chunk's travel-time loads are issued after the waits (they
+        // follow in program order and the compiler may not hoist them)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        run_column<TS, DL, false, true>(problems, pIndex, bx, by, bz, sweep,
+                                        initialize, chunk, T, S, change,
+                                        ring, lane);
+        // publish once every store of the chunk has completed
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_store(q.flags + self, q.epoch, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Zeroes n words (the queue's flags and counters) with vector stores.
+__global__ void __launch_bounds__(256)
+clear_words_kernel(unsigned int *__restrict__ w, int64_t n)
+{
+    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x; i < n;
+         i += (int64_t) gridDim.x*blockDim.x)
+    {
+        w[i] = 0;
     }
 }
 
@@ -992,6 +1186,49 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
 }
 
 template <typename TS>
+hipError_t launch_queued_columns3d(const Problem3D *dProblems,
+                                   const ColumnItem *dItems, int nItems,
+                                   int sweep, int initialize, int chunk, TS *T,
+                                   const TS *S, unsigned int *change,
+                                   const ColumnQueue &q, int waves, int depth,
+                                   hipStream_t stream)
+{
+    if (nItems <= 0) { return hipSuccess; }
+    static_assert(merged_factored<TS>(),
+                  "the work queue runs the factored-box columns in place");
+    const dim3 grid((unsigned) std::max(1, std::min(nItems, waves)));
+    switch (depth)
+    {
+    case 4:
+        hipLaunchKernelGGL((sweep_columns3d_queued<TS, 4>), grid, dim3(64),
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, chunk, T, S, change, q);
+        break;
+    case 8:
+        hipLaunchKernelGGL((sweep_columns3d_queued<TS, 8>), grid, dim3(64),
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, chunk, T, S, change, q);
+        break;
+    case 12:
+        hipLaunchKernelGGL((sweep_columns3d_queued<TS, 12>), grid, dim3(64),
+                           0, stream, dProblems, dItems, nItems, sweep,
+                           initialize, chunk, T, S, change, q);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_words(unsigned int *w, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) { return hipSuccess; }
+    hipLaunchKernelGGL(clear_words_kernel, dim3(grid_for(n, 256, 1024)),
+                       dim3(256), 0, stream, w, n);
+    return hipGetLastError();
+}
+
+template <typename TS>
 hipError_t launch_tile_slowness(int nx, int ny, int nz, const TS *s, TS *b,
                                 hipStream_t stream)
 {
@@ -1103,6 +1340,10 @@ template hipError_t launch_columns3d<float>(
 template hipError_t launch_factored_columns3d<float>(
     const Problem3D *, int, int, int, int, int, int, float *, const float *,
     unsigned int *, hipStream_t);
+template hipError_t launch_queued_columns3d<double>(
+    const Problem3D *, const ColumnItem *, int, int, int, int, double *,
+    const double *, unsigned int *, const ColumnQueue &, int, int,
+    hipStream_t);
 template hipError_t launch_columns3d<double>(
     const Problem3D *, const ColumnItem *, int, int, int, int, int, int,
     double *, const double *, unsigned int *, int, hipStream_t);

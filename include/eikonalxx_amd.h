@@ -87,6 +87,15 @@ typedef enum
                                   one launch per column diagonal         */
 } ekx_kernel;
 
+/* Launch order of the column kernel's sweeps (FP64 fields). */
+typedef enum
+{
+    EKX_LAUNCH_AUTO = 0,
+    EKX_LAUNCH_DIAGONALS = 1,  /* one launch per diagonal bx+by+bz        */
+    EKX_LAUNCH_QUEUE = 2       /* one launch per sweep, chunks from a work
+                                  queue in dependency order               */
+} ekx_launch_order;
+
 /* SolverOptions (include/eikonalxx/solverOptions.hpp:54-97, defaults
  * src/solverOptions.cpp:5-21) plus the device-side choices. */
 typedef struct
@@ -127,6 +136,14 @@ typedef struct
                                       large batches, down to 8 planes for
                                       one grid); results do not depend on
                                       it                                   */
+    int launch_order;              /* column kernel, FP64 fields: 1: one
+                                      launch per diagonal of column chunks;
+                                      2: one launch per sweep whose waves
+                                      take the chunks from a work queue in
+                                      diagonal order and start each once
+                                      its upwind chunks are done; 0
+                                      (default) chooses.  Results do not
+                                      depend on it                         */
 } ekx_solver_options;
 
 /* Geometry3D (include/eikonalxx/geometry3d.hpp:50-160). */

@@ -14,6 +14,7 @@ int main(void)
     FIELD(ekx_solver_options, precision);
     FIELD(ekx_solver_options, chains);
     FIELD(ekx_solver_options, chunk_planes);
+    FIELD(ekx_solver_options, launch_order);
     printf("ekx_solve_stats size %zu\n", sizeof(ekx_solve_stats));
     FIELD(ekx_solve_stats, updates);
     FIELD(ekx_solve_stats, iterations_run);
