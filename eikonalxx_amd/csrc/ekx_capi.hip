@@ -1168,7 +1168,8 @@ struct Engine3D
         EKX_HIP(launch_queued_columns3d<TS>(
             dProblems.ptr, dItems.ptr + ds[0], count, sweep,
             initialize ? 1 : 0, chunkH, T, bricks<TS>(), change.ptr,
-            column_queue(c, epoch), queueWaves, depth(), s));
+            column_queue(c, epoch), std::max(1, queueWaves/nChains), depth(),
+            s));
         if (timed_sweep<TS>(initialize)) { EKX_CHECK(record_timing_event(s)); }
         launches += 1;
         return EKX_OK;

@@ -58,8 +58,8 @@ hipError_t launch_factored_columns3d(const Problem3D *dProblems,
                                      unsigned int *change,
                                      hipStream_t stream);
 // A whole sweep of one chain's items (the diagonal launches' lists,
-// concatenated in diagonal order) in work-queue form: `waves` resident
-// waves take the items in order and start each once its upwind chunks have
+// concatenated in diagonal order) in work-queue form: `waves` waves (the
+// chain's share of the resident ones) take the items in order and start each once its upwind chunks have
 // published completion in q (the factored-box chunks included; FP64
 // fields only).
 template <typename TS>
