@@ -77,9 +77,9 @@ __device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
 
 // ---------------------------------------------------------------------------
 // Source initialisation (Solver3DImpl::initializeTravelTimes,
-// src/solver3d.cpp:21-84): HUGE everywhere, the analytic time at the 8
-// corners of the source cell.  blockIdx.y is the problem.  Covers the whole
-// extent of a field (tile padding and edge records included).
+// src/solver3d.cpp:21-84): HUGE everywhere (a streaming fill over the whole
+// extent of a field, tile padding and edge records included; blockIdx.y is
+// the problem), then the analytic time at the 8 corners of the source cell.
 // ---------------------------------------------------------------------------
 template <typename TS>
 __global__ void __launch_bounds__(256)
@@ -87,47 +87,34 @@ init_travel_times3d(const Problem3D *__restrict__ problems,
                     TS *__restrict__ T)
 {
     const Problem3D &p = problems[blockIdx.y];
-    const int64_t nxy = (int64_t) p.nx*p.ny;
-    // tiled: tile records, then edge records (HUGE as well)
-    const int64_t nTileValues = (int64_t) p.nTX*p.nTY*p.nz*kTileValues;
     TS *Tp = T + p.tOffset;
+    const TS huge = Store<TS>::kHugeCode;
     for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
          i < p.tCount; i += (int64_t) gridDim.x*blockDim.x)
     {
-        int ix, iy, iz;
-        if (p.tiled)
-        {
-            if (i >= nTileValues)
-            {
-                Tp[i] = Store<TS>::kHugeCode;
-                continue;
-            }
-            const int64_t rec = i/kTileValues;
-            const int local = (int) (i - rec*kTileValues);
-            const int tile = (int) (rec/p.nz);
-            iz = (int) (rec - (int64_t) tile*p.nz);
-            const int ty = tile/p.nTX;
-            ix = (tile - ty*p.nTX)*kColumnTile + local % kColumnTile;
-            iy = ty*kColumnTile + local/kColumnTile;
-        }
-        else
-        {
-            iz = (int) (i/nxy);
-            int rem = (int) (i - (int64_t) iz*nxy);
-            iy = rem/p.nx;
-            ix = rem - iy*p.nx;
-        }
-        int ox = ix - p.srcCellX, oy = iy - p.srcCellY, oz = iz - p.srcCellZ;
-        TS v = Store<TS>::kHugeCode;
-        if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1 && oz >= 0 && oz <= 1)
-        {
-            v = Store<TS>::code(
-                (typename Store<TS>::R) analytic_time3d(
-                    ix, iy, iz, p.dx, p.dy, p.dz, p.srcOffX, p.srcOffY,
-                    p.srcOffZ, p.srcSlowness), p);
-        }
-        Tp[i] = v;
+        Tp[i] = huge;
     }
+}
+
+// Then the 8 corners of each problem's source cell (thread 8 k + c:
+// problem k, corner c), in the field's layout.
+template <typename TS>
+__global__ void __launch_bounds__(64)
+init_source_corners3d(const Problem3D *__restrict__ problems, int nProblems,
+                      TS *__restrict__ T)
+{
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= 8*nProblems) { return; }
+    const Problem3D &p = problems[t >> 3];
+    const int c = t & 7;
+    const int ix = p.srcCellX + (c & 1);
+    const int iy = p.srcCellY + ((c >> 1) & 1);
+    const int iz = p.srcCellZ + (c >> 2);
+    if (ix >= p.nx || iy >= p.ny || iz >= p.nz) { return; }
+    T[p.tOffset + node_index(p, ix, iy, iz)] = Store<TS>::code(
+        (typename Store<TS>::R) analytic_time3d(
+            ix, iy, iz, p.dx, p.dy, p.dz, p.srcOffX, p.srcOffY, p.srcOffZ,
+            p.srcSlowness), p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1114,6 +1101,9 @@ hipError_t launch_init3d(const Problem3D *dProblems, int nProblems,
     dim3 grid(grid_for(maxNodes, 256, 4096), (unsigned) nProblems);
     hipLaunchKernelGGL(init_travel_times3d<TS>, grid, dim3(256), 0, stream,
                        dProblems, T);
+    hipLaunchKernelGGL(init_source_corners3d<TS>,
+                       dim3((unsigned) ((8*nProblems + 63)/64)), dim3(64), 0,
+                       stream, dProblems, nProblems, T);
     return hipGetLastError();
 }
 
