@@ -1439,7 +1439,7 @@ struct Engine3D
         for (int pass = 0; pass < 2; ++pass)
         {
             EKX_HIP(hipEventRecord(evStart, stream));
-            EKX_HIP(launch_init3d<TS>(dProblems.ptr, nProb, maxCount, T,
+            EKX_HIP(launch_init3d<TS>(dProblems.ptr, nProb, totalNodes, T,
                                       stream));
             EKX_CHECK(run_segment<TS>(0, T, S, true, 1, st));
             EKX_HIP(hipEventRecord(evStop, stream));
@@ -1580,7 +1580,7 @@ struct Engine3D
         EKX_HIP(hipEventRecord(evStart, stream));
         if (sweeps_f64())
         {
-            EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, maxCount,
+            EKX_HIP(launch_init3d<double>(dProblems.ptr, nProb, totalNodes,
                                           T64.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<double>(T64.ptr, S64.ptr, nIt, st));
@@ -1588,7 +1588,7 @@ struct Engine3D
         }
         else if (fixed32())
         {
-            EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, maxCount,
+            EKX_HIP(launch_init3d<uint32_t>(dProblems.ptr, nProb, totalNodes,
                                             TQ.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<uint32_t>(TQ.ptr, SQ.ptr, itPrimary, st));
@@ -1596,7 +1596,7 @@ struct Engine3D
         }
         else
         {
-            EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, maxCount,
+            EKX_HIP(launch_init3d<float>(dProblems.ptr, nProb, totalNodes,
                                          T32.ptr, stream));
             EKX_HIP(hipEventRecord(evSeg[0], stream));
             EKX_CHECK(run_primary<float>(T32.ptr, S32.ptr, itPrimary, st));

@@ -24,11 +24,12 @@ template <typename TS> constexpr bool merged_factored()
     return sizeof(TS) == 8;
 }
 
-// HUGE everywhere in every problem's field (maxCount: the largest tCount),
-// the analytic times at the source cell's corners.
+// HUGE everywhere in the fields of all problems (totalCount elements from
+// T, the problems' fields back to back), then the analytic times at each
+// source cell's corners.
 template <typename TS>
 hipError_t launch_init3d(const Problem3D *dProblems, int nProblems,
-                         int64_t maxCount, TS *T, hipStream_t stream);
+                         int64_t totalCount, TS *T, hipStream_t stream);
 template <typename TS>
 hipError_t launch_hyperplane3d(const Problem3D *dProblems, int problem,
                                int sweep, int initialize, int level, int kLo,

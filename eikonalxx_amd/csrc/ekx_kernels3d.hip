@@ -77,22 +77,35 @@ __device__ __forceinline__ BoxColumns box_columns(const Problem3D &p,
 
 // ---------------------------------------------------------------------------
 // Source initialisation (Solver3DImpl::initializeTravelTimes,
-// src/solver3d.cpp:21-84): HUGE everywhere (a streaming fill over the whole
-// extent of a field, tile padding and edge records included; blockIdx.y is
-// the problem), then the analytic time at the 8 corners of the source cell.
+// src/solver3d.cpp:21-84): HUGE everywhere (one streaming fill over the
+// fields of all problems, which lie back to back, tile padding and edge
+// records included), then the analytic time at the 8 corners of each
+// problem's source cell.
 // ---------------------------------------------------------------------------
 template <typename TS>
 __global__ void __launch_bounds__(256)
-init_travel_times3d(const Problem3D *__restrict__ problems,
-                    TS *__restrict__ T)
+init_travel_times3d(TS *__restrict__ T, int64_t n)
 {
-    const Problem3D &p = problems[blockIdx.y];
-    TS *Tp = T + p.tOffset;
-    const TS huge = Store<TS>::kHugeCode;
-    for (int64_t i = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
-         i < p.tCount; i += (int64_t) gridDim.x*blockDim.x)
+    // 16-byte stores over the aligned body, element stores for the tail
+    constexpr int kPer = 16/sizeof(TS);
+    union Pack
     {
-        Tp[i] = huge;
+        TS v[kPer];
+        uint4 u;
+    } pack;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) { pack.v[k] = Store<TS>::kHugeCode; }
+    const int64_t nVec = n/kPer;
+    uint4 *V = reinterpret_cast<uint4 *>(T);
+    const int64_t stride = (int64_t) gridDim.x*blockDim.x;
+    const int64_t first = (int64_t) blockIdx.x*blockDim.x + threadIdx.x;
+    for (int64_t i = first; i < nVec; i += stride)
+    {
+        V[i] = pack.u;
+    }
+    for (int64_t i = nVec*kPer + first; i < n; i += stride)
+    {
+        T[i] = Store<TS>::kHugeCode;
     }
 }
 
@@ -1096,11 +1109,12 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap)
 
 template <typename TS>
 hipError_t launch_init3d(const Problem3D *dProblems, int nProblems,
-                         int64_t maxNodes, TS *T, hipStream_t stream)
+                         int64_t totalCount, TS *T, hipStream_t stream)
 {
-    dim3 grid(grid_for(maxNodes, 256, 4096), (unsigned) nProblems);
-    hipLaunchKernelGGL(init_travel_times3d<TS>, grid, dim3(256), 0, stream,
-                       dProblems, T);
+    hipLaunchKernelGGL(init_travel_times3d<TS>,
+                       dim3(grid_for(totalCount/(16/sizeof(TS)) + 1, 256,
+                                     16384)),
+                       dim3(256), 0, stream, T, totalCount);
     hipLaunchKernelGGL(init_source_corners3d<TS>,
                        dim3((unsigned) ((8*nProblems + 63)/64)), dim3(64), 0,
                        stream, dProblems, nProblems, T);
