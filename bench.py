@@ -93,13 +93,14 @@ def cpu_baseline(w, slow, source, threads, iterations):
 
 
 def secondary_workloads(ekx):
-    """The other 3D workloads of the benchmark description, timed in FP64
-    and FP32_POLISH (updates/s of the best of 2 solves after a warmup; max
-    and RMS relative error of FP32_POLISH against FP64 on the same run):
-    one 257 x 257 x 193 grid (12 layers 2000-7000 m/s x (1 + 0.05 G),
-    l = 250 m, radius 5, 3 iterations), and 64 independent problems with
-    96..320 points per axis (h = 20 m, the same model family, one model
-    each).  Models come from the library's GPU generator (seeded)."""
+    """The other workloads of the benchmark description (updates/s of the
+    best of 2 solves after a warmup): one 257 x 257 x 193 grid (12 layers
+    2000-7000 m/s x (1 + 0.05 G), l = 250 m, radius 5, 3 iterations) and 64
+    independent problems with 96..320 points per axis (h = 20 m, the same
+    model family, one model each), in FP64 and FP32_POLISH with the max and
+    RMS relative error of FP32_POLISH against FP64 on the same run (models
+    from the library's GPU generator, seeded); and the 2D 201 x 201
+    constant-velocity case in FP64 against the analytic field."""
     from eikonalxx_amd import workloads
     out = {}
 
@@ -176,6 +177,27 @@ def secondary_workloads(ekx):
     log("64 uneven problems: FP64 %.3e, FP32_POLISH %.3e updates/s"
         % (entry["FP64"]["updates_per_s"],
            entry["FP32_POLISH"]["updates_per_s"]))
+
+    # the 2D configuration: 201 x 201, h = 10 m, 5000 m/s, source
+    # (1005, 995) m, radius 3, 2 iterations; error against |x - xs| / v
+    w2 = workloads.config1_2d()
+    g2 = ekx.Geometry2D(w2["nx"], w2["nz"], w2["h"])
+    s2 = ekx.Solver2D()
+    s2.initialize(ekx.SolverOptions(number_of_sweeps=w2["n_sweeps"],
+                                    factored_radius=w2["radius"]), g2)
+    s2.set_nodal_velocities(np.full(w2["nx"] * w2["nz"], w2["velocity"]))
+    s2.set_source(w2["source"])
+    rate, ms = timed(s2.solve, s2.get_stats)
+    t2 = s2.get_travel_time_field().reshape(w2["nz"], w2["nx"])
+    xg = np.arange(w2["nx"]) * w2["h"] - w2["source"][0]
+    zg = np.arange(w2["nz"]) * w2["h"] - w2["source"][1]
+    exact = np.hypot(xg[None, :], zg[:, None]) / w2["velocity"]
+    mx, rms = rel_error(t2, exact, 1.0 / w2["velocity"], w2["h"])
+    out["2d_201x201"] = {"FP64": {"updates_per_s": rate, "ms_per_solve": ms,
+                                  "max_rel_err_vs_analytic": mx,
+                                  "rms_rel_err_vs_analytic": rms}}
+    log("2D 201x201: FP64 %.3e updates/s, %.3f ms (max rel err vs "
+        "|x - xs|/v %.2e: the first-order scheme)" % (rate, ms, mx))
     return out
 
 
