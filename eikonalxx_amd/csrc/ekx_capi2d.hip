@@ -365,8 +365,8 @@ ekx_status ekx_solver2d_solve(ekx_solver2d *h)
     EKX2_HIP(hipMemcpyAsync(h->dProblem.ptr, &p, sizeof(Problem2D),
                             hipMemcpyHostToDevice, h->stream));
     EKX2_HIP(hipEventRecord(h->ev0, h->stream));
-    EKX2_HIP(launch_solve2d(h->dProblem.ptr, 1, h->T.ptr, h->S.ptr,
-                            h->stream));
+    EKX2_HIP(launch_solve2d(h->dProblem.ptr, 1, h->geometry.nz, h->T.ptr,
+                            h->S.ptr, h->stream));
     EKX2_HIP(hipEventRecord(h->ev1, h->stream));
     const int ns = (int) h->stations.size();
     if (ns > 0)

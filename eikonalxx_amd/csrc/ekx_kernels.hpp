@@ -140,8 +140,12 @@ hipError_t launch_layered_velocity(int nx, int ny, int nz, int nLayers,
                                    double mean, double sd, const double *g,
                                    double *v, hipStream_t stream);
 
+// The whole 2D solve, one 256-thread workgroup per problem: the latest
+// levels in LDS when every problem has at most 256 rows (maxNz), else
+// memory only, with a barrier per level.
 hipError_t launch_solve2d(const Problem2D *dProblems, int nProblems,
-                          double *T, const double *S, hipStream_t stream);
+                          int maxNz, double *T, const double *S,
+                          hipStream_t stream);
 hipError_t launch_stations2d(const Problem2D *dProblem, int n,
                              const double *xOff, const double *zOff,
                              const int *cellX, const int *cellZ,

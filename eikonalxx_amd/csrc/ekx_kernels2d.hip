@@ -9,8 +9,10 @@
 // Solver2DSweep::updateFSM (include/private/solver2d.hpp:203-327).
 //
 // One workgroup owns one problem for the whole solve; levels are separated
-// by workgroup barriers.  2D grids are small (config of the reference's own
-// 2D tests: 201 x 201), so one launch covers the entire solve.
+// by workgroup barriers, and for grids of up to 256 rows the last three
+// levels of a sweep live in LDS.  2D grids are small (config of the
+// reference's own 2D tests: 201 x 201), so one launch covers the entire
+// solve.
 #include "ekx_kernels.hpp"
 
 namespace ekx
@@ -288,6 +290,140 @@ __device__ void sweep2d_block(const Problem2D &p, int sweep, int initialize,
     }
 }
 
+// One sweep of one problem by one 256-thread workgroup with a node per
+// thread and level, for grids of at most kLdsRows2d rows (nz): the three
+// latest levels of the sweep stay in LDS, indexed by the permuted row jz,
+// so that a node's upwind neighbours (levels L - 1 and L - 2) are LDS
+// reads behind a barrier that waits for LDS only, while the node's old
+// time and cells, which no node of the sweep writes before it, are loaded
+// a level ahead (after the level's own store, so that waiting for them
+// waits for nothing newer) and stay in flight across the barrier.  Nodes of a level
+// that the sweep leaves alone (the boundary row and column, the part
+// outside an initialization sweep's range) enter the buffers with their
+// stored times; every access is unconditional and in bounds (threads past
+// the level's last row repeat it and store the same value), so the
+// compiler counts the prefetch loads exactly.
+constexpr int kLdsRows2d = 256;
+
+__device__ void sweep2d_lds(const Problem2D &p, int sweep, int initialize,
+                            double *__restrict__ T,
+                            const double *__restrict__ S,
+                            double (*lev)[kLdsRows2d])
+{
+    const int tid = threadIdx.x;
+    const int nx = p.nx, nz = p.nz, ncx = nx - 1, ncz = nz - 1;
+    const int negX = sweep & 1, negZ = (sweep >> 1) & 1;
+    const int signX = negX ? -1 : 1, signZ = negZ ? -1 : 1;
+    const int ixShift = negX ? 1 : -1, izShift = negZ ? 1 : -1;
+    int jx0 = 1, jz0 = 1;
+    if (initialize)
+    {
+        // getLoopLimits (init) 2D, solverUtilities2d.hpp:1004-1053
+        jx0 = negX ? max(nx - 2 - p.srcCellX, 1) : max(p.srcCellX, 1);
+        jz0 = negZ ? max(nz - 2 - p.srcCellZ, 1) : max(p.srcCellZ, 1);
+    }
+    const int lastLevel = (nx - 1) + (nz - 1);
+    const int firstLevel = max(0, jx0 + jz0 - 2);
+    auto node = [&](int level, int &jx, int &jz)
+    {
+        const int lo = max(0, level - (nx - 1)), hi = min(nz - 1, level);
+        jz = min(lo + tid, hi);
+        jx = level - jz;
+    };
+    auto fetch = [&](int level, double &f0, double &f1, double &f2,
+                     double &f3)
+    {
+        int jx, jz;
+        node(level, jx, jz);
+        const int ix = negX ? nx - 1 - jx : jx;
+        const int iz = negZ ? nz - 1 - jz : jz;
+        f0 = T[iz*nx + ix];
+        // gridToSurroundingSlowness (solverUtilities2d.hpp:1279-1370):
+        // home (jx-1, jz-1), offsets clamped
+        const int pcxH = max(jx - 1, 0), pcxO = min(jx, ncx - 1);
+        const int pczH = max(jz - 1, 0), pczO = min(jz, ncz - 1);
+        const int cxH = negX ? ncx - 1 - pcxH : pcxH;
+        const int cxO = negX ? ncx - 1 - pcxO : pcxO;
+        const int czH = negZ ? ncz - 1 - pczH : pczH;
+        const int czO = negZ ? ncz - 1 - pczO : pczO;
+        f1 = S[czH*ncx + cxH];
+        f2 = S[czH*ncx + cxO];
+        f3 = S[czO*ncx + cxH];
+    };
+    double t0, s0, s1, s3;
+    fetch(firstLevel, t0, s0, s1, s3);
+    for (int level = firstLevel; level <= lastLevel; ++level)
+    {
+        int jx, jz;
+        node(level, jx, jz);
+        const int ix = negX ? nx - 1 - jx : jx;
+        const int iz = negZ ? nz - 1 - jz : jz;
+        const int jzU = max(jz - 1, 0);
+        const double t1 = lev[(level + 2) % 3][jz];     // (jx - 1, jz)
+        const double t3 = lev[(level + 2) % 3][jzU];    // (jx, jz - 1)
+        const double t2 = lev[(level + 1) % 3][jzU];    // (jx - 1, jz - 1)
+        double v = t0;
+        if (jx >= jx0 && jz >= jz0)
+        {
+            const double tUpd = p.uniform
+                ? fd2d_uniform(p, ix, iz, signX, signZ, ixShift, izShift,
+                               s0, s1, s3, t1, t2, t3)
+                : fd2d_nonuniform(p, ix, iz, signX, signZ, ixShift, izShift,
+                                  s0, s1, s3, t1, t2, t3);
+            v = tUpd < v ? tUpd : v;
+        }
+        T[iz*nx + ix] = v;
+        lev[level % 3][jz] = v;
+        // the next level's node, in flight across the barrier (the last
+        // level repeats itself: in bounds, unused)
+        fetch(min(level + 1, lastLevel), t0, s0, s1, s3);
+        // LDS writes done, then the barrier (global accesses stay in
+        // flight: the field is read back only after the sweep)
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_s_barrier();
+    }
+    // the next sweep reads this one's stores from memory
+    __syncthreads();
+}
+
+// The whole solve of problem blockIdx.x by one 256-thread workgroup with
+// LDS level buffers (nz <= kLdsRows2d).
+__global__ void __launch_bounds__(256)
+solve2d_lds_kernel(const Problem2D *__restrict__ problems,
+                   double *__restrict__ T, const double *__restrict__ S)
+{
+    __shared__ double lev[3][kLdsRows2d];
+    const Problem2D &p = problems[blockIdx.x];
+    double *Tp = T + p.tOffset;
+    const double *Sp = S + p.sOffset;
+    const int n = p.nx*p.nz;
+    // Solver2DImpl::initializeTravelTimes (src/solver2d.cpp:61-105)
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        int iz = i/p.nx, ix = i - iz*p.nx;
+        int ox = ix - p.srcCellX, oz = iz - p.srcCellZ;
+        double v = kHuge;
+        if (ox >= 0 && ox <= 1 && oz >= 0 && oz <= 1)
+        {
+            v = analytic_time2d(ix, iz, p.dx, p.dz, p.srcOffX, p.srcOffZ,
+                                p.srcSlowness);
+        }
+        Tp[i] = v;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 4; ++sweep)
+    {
+        sweep2d_lds(p, sweep, 1, Tp, Sp, lev);
+    }
+    for (int k = 0; k < p.nSweeps; ++k)
+    {
+        for (int sweep = 0; sweep < 4; ++sweep)
+        {
+            sweep2d_lds(p, sweep, 0, Tp, Sp, lev);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(1024)
 solve2d_kernel(const Problem2D *__restrict__ problems, double *__restrict__ T,
                const double *__restrict__ S)
@@ -396,10 +532,19 @@ nodal_to_cell2d(int nx, int nz, const double *__restrict__ v, int ordering,
 }
 
 hipError_t launch_solve2d(const Problem2D *dProblems, int nProblems,
-                          double *T, const double *S, hipStream_t stream)
+                          int maxNz, double *T, const double *S,
+                          hipStream_t stream)
 {
-    hipLaunchKernelGGL(solve2d_kernel, dim3((unsigned) nProblems), dim3(256),
-                       0, stream, dProblems, T, S);
+    if (maxNz <= kLdsRows2d)
+    {
+        hipLaunchKernelGGL(solve2d_lds_kernel, dim3((unsigned) nProblems),
+                           dim3(256), 0, stream, dProblems, T, S);
+    }
+    else
+    {
+        hipLaunchKernelGGL(solve2d_kernel, dim3((unsigned) nProblems),
+                           dim3(256), 0, stream, dProblems, T, S);
+    }
     return hipGetLastError();
 }
 
