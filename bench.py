@@ -100,7 +100,8 @@ def secondary_workloads(ekx):
     model family, one model each), in FP64 and FP32_POLISH with the max and
     RMS relative error of FP32_POLISH against FP64 on the same run (models
     from the library's GPU generator, seeded); and the 2D 201 x 201
-    constant-velocity case in FP64 against the analytic field."""
+    constant-velocity case in FP64 against the analytic field, alone and
+    with 1024 surface sources in one batch."""
     from eikonalxx_amd import workloads
     out = {}
 
@@ -198,6 +199,19 @@ def secondary_workloads(ekx):
                                   "rms_rel_err_vs_analytic": rms}}
     log("2D 201x201: FP64 %.3e updates/s, %.3f ms (max rel err vs "
         "|x - xs|/v %.2e: the first-order scheme)" % (rate, ms, mx))
+    # the same model with 1024 surface sources in one batched launch
+    rng2 = np.random.default_rng(1001)
+    slow2 = np.full((w2["nx"] - 1) * (w2["nz"] - 1), 1.0 / w2["velocity"])
+    xs2 = rng2.uniform(0.0, (w2["nx"] - 1) * w2["h"], 1024)
+    b2 = ekx.Batch2D()
+    b2.initialize(ekx.SolverOptions(number_of_sweeps=w2["n_sweeps"],
+                                    factored_radius=w2["radius"]),
+                  [(g2, slow2, (float(x), 0.0)) for x in xs2])
+    rate, ms = timed(b2.solve, b2.get_stats)
+    out["2d_201x201_batch1024"] = {"FP64": {"updates_per_s": rate,
+                                            "ms_per_solve": ms}}
+    log("2D 201x201, 1024 sources in one batch: FP64 %.3e updates/s, "
+        "%.2f ms" % (rate, ms))
     return out
 
 

@@ -24,10 +24,10 @@ LIBRARY_PATH = os.environ.get("EKX_LIBRARY_PATH",
                               os.path.join(_HERE, "libeikonalxx_amd.so"))
 
 __all__ = ["SolverAlgorithm", "Precision", "Schedule", "Kernel",
-           "LaunchOrder", "Ordering",
-           "SolverOptions", "Geometry3D", "Geometry2D", "Solver3D",
-           "Batch3D", "Solver2D", "SolveStats", "library", "device_count",
-           "synchronize", "DeviceError"]
+           "LaunchOrder", "Ordering", "SolverOptions", "Geometry3D",
+           "Geometry2D", "Solver3D", "Batch3D", "Batch2D", "Solver2D",
+           "SolveStats", "library", "device_count", "synchronize",
+           "DeviceError"]
 
 
 class SolverAlgorithm(enum.IntEnum):
@@ -132,6 +132,12 @@ class _Problem3D(ctypes.Structure):
                 ("source", ctypes.c_double * 3)]
 
 
+class _Problem2D(ctypes.Structure):
+    _fields_ = [("geometry", _Geometry2D),
+                ("slowness", ctypes.POINTER(ctypes.c_double)),
+                ("source", ctypes.c_double * 2)]
+
+
 class _LayeredGaussian(ctypes.Structure):
     _fields_ = [("n_layers", ctypes.c_int),
                 ("layer_velocity", ctypes.POINTER(ctypes.c_double)),
@@ -177,6 +183,9 @@ EXPORTED_SYMBOLS = [
     "ekx_solver2d_get_travel_time_gradient_field",
     "ekx_model3d_layered_gaussian", "ekx_solver3d_set_layered_gaussian_model",
     "ekx_solver3d_autotune", "ekx_batch3d_autotune",
+    "ekx_batch2d_create", "ekx_batch2d_destroy", "ekx_batch2d_initialize",
+    "ekx_batch2d_solve", "ekx_batch2d_get_travel_times",
+    "ekx_batch2d_get_stats",
 ]
 
 _lib = None
@@ -209,12 +218,17 @@ def _declare(L):
     L.ekx_solver_options_default.argtypes = [ctypes.POINTER(_Options)]
     L.ekx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     for name in ("ekx_solver3d_destroy", "ekx_batch3d_destroy",
-                 "ekx_solver2d_destroy"):
+                 "ekx_solver2d_destroy", "ekx_batch2d_destroy"):
         getattr(L, name).restype = None
         getattr(L, name).argtypes = [_P]
     for name in ("ekx_solver3d_create", "ekx_batch3d_create",
-                 "ekx_solver2d_create"):
+                 "ekx_solver2d_create", "ekx_batch2d_create"):
         getattr(L, name).argtypes = [ctypes.POINTER(_P)]
+    L.ekx_batch2d_initialize.argtypes = [_P, ctypes.POINTER(_Options), _I,
+                                         ctypes.POINTER(_Problem2D)]
+    L.ekx_batch2d_solve.argtypes = [_P]
+    L.ekx_batch2d_get_travel_times.argtypes = [_P, _I, _I64, _PD]
+    L.ekx_batch2d_get_stats.argtypes = [_P, ctypes.POINTER(SolveStats)]
     L.ekx_solver3d_initialize.argtypes = [_P, ctypes.POINTER(_Options),
                                           ctypes.POINTER(_Geometry3D)]
     L.ekx_solver3d_set_slowness.argtypes = [_P, _I64, _PD]
@@ -714,6 +728,65 @@ class Batch3D:
         s = _P()
         _check(self._L.ekx_batch3d_get_stream(self._h, ctypes.byref(s)))
         return s.value
+
+
+class Batch2D:
+    """Many independent 2D solves in one launch (no reference counterpart;
+    problem i's field equals a Solver2D's with the same inputs).
+    ``problems`` is a list of (Geometry2D, cell slowness array, (x, z));
+    problems passing the same slowness array object share one device copy
+    of the model."""
+
+    def __init__(self):
+        self._L = library()
+        self._h = _P()
+        _check(self._L.ekx_batch2d_create(ctypes.byref(self._h)))
+        self._keep = []
+        self._geoms = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.ekx_batch2d_destroy(h)
+            self._h = None
+
+    def initialize(self, options, problems):
+        arr = (_Problem2D * len(problems))()
+        keep, seen = [], {}
+        for i, (geom, slow, src) in enumerate(problems):
+            key = id(slow)
+            if key not in seen:
+                s = np.ascontiguousarray(slow, dtype=np.float64).reshape(-1)
+                if s.size != geom.n_cells:
+                    raise ValueError("problem %d: slowness has %d cells, "
+                                     "geometry needs %d"
+                                     % (i, s.size, geom.n_cells))
+                seen[key] = s
+                keep.append(s)
+            s = seen[key]
+            arr[i].geometry = geom._c()
+            arr[i].slowness = _dp(s)
+            arr[i].source[0], arr[i].source[1] = src
+        _check(self._L.ekx_batch2d_initialize(self._h,
+                                              ctypes.byref(options._o),
+                                              len(problems), arr))
+        self._keep = keep
+        self._geoms = [p[0] for p in problems]
+
+    def solve(self):
+        _check(self._L.ekx_batch2d_solve(self._h))
+
+    def get_travel_time_field(self, problem):
+        g = self._geoms[problem]
+        out = np.empty(g.n_grid, dtype=np.float64)
+        _check(self._L.ekx_batch2d_get_travel_times(self._h, int(problem),
+                                                    out.size, _dp(out)))
+        return out
+
+    def get_stats(self):
+        st = SolveStats()
+        _check(self._L.ekx_batch2d_get_stats(self._h, ctypes.byref(st)))
+        return st
 
 
 class Solver2D:

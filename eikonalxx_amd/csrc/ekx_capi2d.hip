@@ -110,6 +110,61 @@ ekx_status locate2d(const ekx_geometry2d &g, double x, double z, Loc2D &l,
 
 } // namespace
 
+// Solver2DSweep::initialize (include/private/solver2d.hpp:29-46): the
+// problem constants of a geometry.
+static Problem2D problem2d(const ekx_geometry2d &g, const ekx_solver_options &o)
+{
+    Problem2D p{};
+    p.nx = g.nx;
+    p.nz = g.nz;
+    p.radius = o.factored_radius;
+    p.nSweeps = o.number_of_sweeps;
+    p.uniform = std::abs(g.dx - g.dz) <
+                std::numeric_limits<double>::epsilon()*100 ? 1 : 0;
+    p.dx = g.dx;
+    p.dz = g.dz;
+    p.dx_dz = g.dx/g.dz;
+    p.dz_dx = g.dz/g.dx;
+    p.cosTheta = g.dx/std::hypot(g.dx, g.dz);
+    p.sinTheta = g.dz/std::hypot(g.dx, g.dz);
+    return p;
+}
+
+// Geometry2D setters (src/geometry2d.cpp)
+static ekx_status check_geometry2d(const ekx_geometry2d &g)
+{
+    if (g.nx < 3 || g.nz < 3)
+    {
+        return fail2(EKX_EINVAL, "Number of grid points must be at least 3");
+    }
+    if (!(g.dx > 0) || !(g.dz > 0))
+    {
+        return fail2(EKX_EINVAL, "Grid spacing must be positive");
+    }
+    if ((int64_t) g.nx*g.nz > (int64_t) std::numeric_limits<int>::max())
+    {
+        return fail2(EKX_EINVAL, "grid too large");
+    }
+    return EKX_OK;
+}
+
+// Local updates of one solve: 4 initialization sweeps over the source
+// quadrants, then nSweeps x 4 refinement sweeps.
+static int64_t updates2d(const Problem2D &p)
+{
+    int64_t u = 0;
+    for (int s = 0; s < 4; ++s)
+    {
+        int negX = s & 1, negZ = (s >> 1) & 1;
+        int jx0 = negX ? std::max(p.nx - 2 - p.srcCellX, 1)
+                       : std::max(p.srcCellX, 1);
+        int jz0 = negZ ? std::max(p.nz - 2 - p.srcCellZ, 1)
+                       : std::max(p.srcCellZ, 1);
+        u += (int64_t) (p.nx - jx0)*(p.nz - jz0);
+    }
+    return u + (int64_t) p.nSweeps*4*(p.nx - 1)*(p.nz - 1);
+}
+
 struct ekx_solver2d
 {
     ekx_solver_options opt{};
@@ -174,19 +229,7 @@ ekx_status ekx_solver2d_initialize(ekx_solver2d *h,
     {
         return fail2(EKX_EINVAL, "number_of_sweeps must be in [0, 65535]");
     }
-    // Geometry2D setters (src/geometry2d.cpp)
-    if (g->nx < 3 || g->nz < 3)
-    {
-        return fail2(EKX_EINVAL, "Number of grid points must be at least 3");
-    }
-    if (!(g->dx > 0) || !(g->dz > 0))
-    {
-        return fail2(EKX_EINVAL, "Grid spacing must be positive");
-    }
-    if ((int64_t) g->nx*g->nz > (int64_t) std::numeric_limits<int>::max())
-    {
-        return fail2(EKX_EINVAL, "grid too large");
-    }
+    EKX2_CHECK(check_geometry2d(*g));
     h->opt = *options;
     h->geometry = *g;
     if (!h->stream)
@@ -196,20 +239,7 @@ ekx_status ekx_solver2d_initialize(ekx_solver2d *h,
         EKX2_HIP(hipEventCreate(&h->ev1));
     }
     Problem2D &p = h->problem;
-    p = Problem2D{};
-    p.nx = g->nx;
-    p.nz = g->nz;
-    p.radius = options->factored_radius;
-    p.nSweeps = options->number_of_sweeps;
-    // Solver2DSweep::initialize (include/private/solver2d.hpp:29-46)
-    p.uniform = std::abs(g->dx - g->dz) <
-                std::numeric_limits<double>::epsilon()*100 ? 1 : 0;
-    p.dx = g->dx;
-    p.dz = g->dz;
-    p.dx_dz = g->dx/g->dz;
-    p.dz_dx = g->dz/g->dx;
-    p.cosTheta = g->dx/std::hypot(g->dx, g->dz);
-    p.sinTheta = g->dz/std::hypot(g->dx, g->dz);
+    p = problem2d(*g, *options);
     p.tOffset = 0;
     p.sOffset = 0;
     EKX2_CHECK(h->S.allocate((size_t) (g->nx - 1)*(g->nz - 1)));
@@ -387,17 +417,7 @@ ekx_status ekx_solver2d_solve(ekx_solver2d *h)
     float ms = 0;
     EKX2_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     ekx_solve_stats st{};
-    // updates: 4 init sweeps over the source quadrants + nSweeps x 4
-    for (int s = 0; s < 4; ++s)
-    {
-        int negX = s & 1, negZ = (s >> 1) & 1;
-        int jx0 = negX ? std::max(p.nx - 2 - p.srcCellX, 1)
-                       : std::max(p.srcCellX, 1);
-        int jz0 = negZ ? std::max(p.nz - 2 - p.srcCellZ, 1)
-                       : std::max(p.srcCellZ, 1);
-        st.updates += (int64_t) (p.nx - jx0)*(p.nz - jz0);
-    }
-    st.updates += (int64_t) p.nSweeps*4*(p.nx - 1)*(p.nz - 1);
+    st.updates = updates2d(p);
     st.updates_fp64 = st.updates;
     st.kernel_launches = 1;
     st.iterations_run = p.nSweeps;
@@ -492,6 +512,219 @@ ekx_status ekx_solver2d_get_stats(const ekx_solver2d *h,
                                   ekx_solve_stats *stats)
 {
     EKX2_CHECK(require2d(h));
+    if (!stats) { return fail2(EKX_EINVAL, "stats is NULL"); }
+    *stats = h->stats;
+    return EKX_OK;
+}
+
+} // extern "C"
+
+// Many independent 2D problems solved by one launch, one workgroup each
+// (no reference counterpart; each problem's field equals that of a
+// Solver2D with the same geometry, model and source).
+struct ekx_batch2d
+{
+    ekx_solver_options opt{};
+    std::vector<ekx_geometry2d> geoms;
+    std::vector<Problem2D> problems;
+    Buf<double> S, T;
+    Buf<Problem2D> dProblems;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int maxNz = 0;
+    bool initialized = false, haveField = false;
+    ekx_solve_stats stats{};
+    ~ekx_batch2d()
+    {
+        if (ev0) { (void) hipEventDestroy(ev0); }
+        if (ev1) { (void) hipEventDestroy(ev1); }
+        if (stream) { (void) hipStreamDestroy(stream); }
+    }
+};
+
+static ekx_status require_batch2d(const ekx_batch2d *h)
+{
+    if (!h) { return fail2(EKX_EINVAL, "batch is NULL"); }
+    if (!h->initialized)
+    {
+        return fail2(EKX_ESTATE, "Class not initialized");
+    }
+    return EKX_OK;
+}
+
+extern "C" {
+
+ekx_status ekx_batch2d_create(ekx_batch2d **batch)
+{
+    if (!batch) { return fail2(EKX_EINVAL, "batch is NULL"); }
+    *batch = new (std::nothrow) ekx_batch2d();
+    if (!*batch) { return fail2(EKX_ENOMEM, "out of host memory"); }
+    return EKX_OK;
+}
+
+void ekx_batch2d_destroy(ekx_batch2d *batch) { delete batch; }
+
+ekx_status ekx_batch2d_initialize(ekx_batch2d *h,
+                                  const ekx_solver_options *options, int n,
+                                  const ekx_problem2d *in)
+{
+    if (!h) { return fail2(EKX_EINVAL, "batch is NULL"); }
+    h->initialized = h->haveField = false;
+    if (!options || !in || n < 1)
+    {
+        return fail2(EKX_EINVAL, "options and at least one problem needed");
+    }
+    if (options->number_of_sweeps < 0 || options->number_of_sweeps > 65535)
+    {
+        return fail2(EKX_EINVAL, "number_of_sweeps must be in [0, 65535]");
+    }
+    if (!h->stream)
+    {
+        EKX2_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        EKX2_HIP(hipEventCreate(&h->ev0));
+        EKX2_HIP(hipEventCreate(&h->ev1));
+    }
+    h->opt = *options;
+    h->geoms.assign(n, ekx_geometry2d{});
+    h->problems.assign(n, Problem2D{});
+    // models: one device copy per distinct host pointer
+    std::vector<const double *> models;
+    std::vector<int64_t> modelOffset;
+    int64_t tTotal = 0, sTotal = 0;
+    h->maxNz = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        const ekx_geometry2d &g = in[i].geometry;
+        EKX2_CHECK(check_geometry2d(g));
+        if (!in[i].slowness)
+        {
+            return fail2(EKX_EINVAL, "problem " + std::to_string(i) +
+                                         ": slowness is NULL");
+        }
+        const int64_t nCell = (int64_t) (g.nx - 1)*(g.nz - 1);
+        for (int64_t c = 0; c < nCell; ++c)
+        {
+            if (!(in[i].slowness[c] > 0))
+            {
+                return fail2(EKX_EINVAL, "problem " + std::to_string(i) +
+                                             ": slowness must be positive");
+            }
+        }
+        Loc2D src;
+        EKX2_CHECK(locate2d(g, in[i].source[0], in[i].source[1], src,
+                            "source"));
+        Problem2D p = problem2d(g, *options);
+        p.srcCellX = src.cell[0];
+        p.srcCellZ = src.cell[1];
+        p.srcOffX = src.off[0];
+        p.srcOffZ = src.off[1];
+        p.srcSlowness = in[i].slowness[(size_t) src.cell[1]*(g.nx - 1) +
+                                       src.cell[0]];
+        size_t m = std::find(models.begin(), models.end(), in[i].slowness)
+                 - models.begin();
+        if (m == models.size())
+        {
+            models.push_back(in[i].slowness);
+            modelOffset.push_back(sTotal);
+            sTotal += nCell;
+        }
+        else
+        {
+            // a shared model needs the same grid
+            for (int j = 0; j < i; ++j)
+            {
+                if (in[j].slowness != in[i].slowness) { continue; }
+                if (in[j].geometry.nx != g.nx || in[j].geometry.nz != g.nz)
+                {
+                    return fail2(EKX_EINVAL, "problems sharing a model must "
+                                             "share its grid");
+                }
+                break;
+            }
+        }
+        p.sOffset = modelOffset[m];
+        p.tOffset = tTotal;
+        tTotal += (int64_t) g.nx*g.nz;
+        h->geoms[i] = g;
+        h->problems[i] = p;
+        h->maxNz = std::max(h->maxNz, g.nz);
+    }
+    EKX2_CHECK(h->S.allocate((size_t) sTotal));
+    EKX2_CHECK(h->T.allocate((size_t) tTotal));
+    EKX2_CHECK(h->dProblems.allocate((size_t) n));
+    for (size_t m = 0; m < models.size(); ++m)
+    {
+        int64_t nCell = 0;
+        for (int i = 0; i < n; ++i)
+        {
+            if (in[i].slowness == models[m])
+            {
+                nCell = (int64_t) (in[i].geometry.nx - 1)*
+                        (in[i].geometry.nz - 1);
+                break;
+            }
+        }
+        EKX2_HIP(hipMemcpyAsync(h->S.ptr + modelOffset[m], models[m],
+                                (size_t) nCell*sizeof(double),
+                                hipMemcpyHostToDevice, h->stream));
+    }
+    EKX2_HIP(hipMemcpyAsync(h->dProblems.ptr, h->problems.data(),
+                            (size_t) n*sizeof(Problem2D),
+                            hipMemcpyHostToDevice, h->stream));
+    EKX2_HIP(hipStreamSynchronize(h->stream));
+    h->initialized = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_batch2d_solve(ekx_batch2d *h)
+{
+    EKX2_CHECK(require_batch2d(h));
+    const int n = (int) h->problems.size();
+    EKX2_HIP(hipEventRecord(h->ev0, h->stream));
+    EKX2_HIP(launch_solve2d(h->dProblems.ptr, n, h->maxNz, h->T.ptr,
+                            h->S.ptr, h->stream));
+    EKX2_HIP(hipEventRecord(h->ev1, h->stream));
+    EKX2_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    EKX2_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    ekx_solve_stats st{};
+    for (const Problem2D &p : h->problems) { st.updates += updates2d(p); }
+    st.updates_fp64 = st.updates;
+    st.kernel_launches = 1;
+    st.iterations_run = h->opt.number_of_sweeps;
+    st.last_solve_ms = ms;
+    h->stats = st;
+    h->haveField = true;
+    return EKX_OK;
+}
+
+ekx_status ekx_batch2d_get_travel_times(const ekx_batch2d *h, int problem,
+                                        int64_t n, double *out)
+{
+    EKX2_CHECK(require_batch2d(h));
+    if (problem < 0 || problem >= (int) h->problems.size())
+    {
+        return fail2(EKX_EINVAL, "problem index out of range");
+    }
+    if (!h->haveField)
+    {
+        return fail2(EKX_ESTATE, "Travel time field not yet computed");
+    }
+    const Problem2D &p = h->problems[problem];
+    const int64_t nGrid = (int64_t) p.nx*p.nz;
+    if (n != nGrid || !out)
+    {
+        return fail2(EKX_EINVAL, "output must hold the " +
+                                     std::to_string(nGrid) + " grid points");
+    }
+    EKX2_HIP(hipMemcpy(out, h->T.ptr + p.tOffset, (size_t) n*sizeof(double),
+                       hipMemcpyDeviceToHost));
+    return EKX_OK;
+}
+
+ekx_status ekx_batch2d_get_stats(const ekx_batch2d *h, ekx_solve_stats *stats)
+{
+    EKX2_CHECK(require_batch2d(h));
     if (!stats) { return fail2(EKX_EINVAL, "stats is NULL"); }
     *stats = h->stats;
     return EKX_OK;

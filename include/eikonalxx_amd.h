@@ -466,6 +466,33 @@ EKX_API ekx_status ekx_solver2d_get_travel_time_gradient_field(
 EKX_API ekx_status ekx_solver2d_get_stats(const ekx_solver2d *solver,
                                           ekx_solve_stats *stats);
 
+/* Many independent 2D problems (geometry, model, source each; problems
+ * passing the same slowness pointer share one device copy of it) solved
+ * by one launch, one workgroup per problem.  No reference counterpart:
+ * problem i's field equals that of a Solver2D (ekx_solver2d_*) with the
+ * same inputs. */
+typedef struct
+{
+    ekx_geometry2d geometry;
+    const double *slowness;   /* host, (nx-1)(nz-1) cells, x fastest */
+    double source[2];         /* absolute (x, z) in m                */
+} ekx_problem2d;
+
+typedef struct ekx_batch2d ekx_batch2d;
+
+EKX_API ekx_status ekx_batch2d_create(ekx_batch2d **batch);
+EKX_API void ekx_batch2d_destroy(ekx_batch2d *batch);
+EKX_API ekx_status ekx_batch2d_initialize(ekx_batch2d *batch,
+                                          const ekx_solver_options *options,
+                                          int n_problems,
+                                          const ekx_problem2d *problems);
+EKX_API ekx_status ekx_batch2d_solve(ekx_batch2d *batch);
+EKX_API ekx_status ekx_batch2d_get_travel_times(const ekx_batch2d *batch,
+                                                int problem, int64_t n,
+                                                double *out);
+EKX_API ekx_status ekx_batch2d_get_stats(const ekx_batch2d *batch,
+                                         ekx_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

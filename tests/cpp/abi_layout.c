@@ -26,6 +26,9 @@ int main(void)
     printf("ekx_geometry3d size %zu\n", sizeof(ekx_geometry3d));
     FIELD(ekx_geometry3d, dx);
     FIELD(ekx_geometry3d, z0);
+    printf("ekx_problem2d size %zu\n", sizeof(ekx_problem2d));
+    FIELD(ekx_problem2d, slowness);
+    FIELD(ekx_problem2d, source);
     printf("ekx_layered_gaussian size %zu\n", sizeof(ekx_layered_gaussian));
     FIELD(ekx_layered_gaussian, layer_velocity);
     FIELD(ekx_layered_gaussian, seed);

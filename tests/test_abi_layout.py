@@ -36,6 +36,7 @@ def layout(tmp_path_factory):
     ("ekx_solve_stats", ekx.SolveStats),
     ("ekx_geometry3d", ekx._Geometry3D),
     ("ekx_layered_gaussian", ekx._LayeredGaussian),
+    ("ekx_problem2d", ekx._Problem2D),
 ])
 def test_struct_layout_matches(layout, cname, pytype):
     sizes, offsets = layout
