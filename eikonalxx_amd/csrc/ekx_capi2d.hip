@@ -602,14 +602,6 @@ ekx_status ekx_batch2d_initialize(ekx_batch2d *h,
                                          ": slowness is NULL");
         }
         const int64_t nCell = (int64_t) (g.nx - 1)*(g.nz - 1);
-        for (int64_t c = 0; c < nCell; ++c)
-        {
-            if (!(in[i].slowness[c] > 0))
-            {
-                return fail2(EKX_EINVAL, "problem " + std::to_string(i) +
-                                             ": slowness must be positive");
-            }
-        }
         Loc2D src;
         EKX2_CHECK(locate2d(g, in[i].source[0], in[i].source[1], src,
                             "source"));
@@ -624,6 +616,16 @@ ekx_status ekx_batch2d_initialize(ekx_batch2d *h,
                  - models.begin();
         if (m == models.size())
         {
+            // a new model: every cell positive (checked once per model)
+            for (int64_t c = 0; c < nCell; ++c)
+            {
+                if (!(in[i].slowness[c] > 0))
+                {
+                    return fail2(EKX_EINVAL, "problem " + std::to_string(i) +
+                                                 ": slowness must be "
+                                                 "positive");
+                }
+            }
             models.push_back(in[i].slowness);
             modelOffset.push_back(sTotal);
             sTotal += nCell;
