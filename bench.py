@@ -518,6 +518,13 @@ def main():
     pool = within if within else modes
     head = max(pool, key=lambda m: results[m]["updates_per_s"])
     hr = results[head]
+    # the same choice under the RMS reading of the error bar (reported
+    # beside the headline, which keeps the max-error reading)
+    within_rms = [m for m in modes
+                  if results[m]["rms_rel_err_vs_fp64"] is not None
+                  and results[m]["rms_rel_err_vs_fp64"] <= ERROR_BAR]
+    head_rms = (max(within_rms, key=lambda m: results[m]["updates_per_s"])
+                if within_rms else None)
 
     # ---- the plain FP64 solve (one source) ---------------------------------
     plain_rate = None
@@ -592,6 +599,13 @@ def main():
             "headline_rule": "fastest mode with max rel err <= %g on the "
                              "sampled sources" % ERROR_BAR,
             "within_error_bar": head in within,
+            "fastest_within_rms_error_bar": None if head_rms is None else {
+                "precision": head_rms,
+                "updates_per_s": results[head_rms]["updates_per_s"],
+                "max_rel_err_vs_fp64":
+                    results[head_rms]["max_rel_err_vs_fp64"],
+                "rms_rel_err_vs_fp64":
+                    results[head_rms]["rms_rel_err_vs_fp64"]},
             "updates_per_step": hr["updates_per_step"],
             "updates_fp64_per_step": hr["updates_fp64_per_step"],
             "updates_factored_per_step": hr["updates_factored_per_step"],
